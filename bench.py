@@ -1,0 +1,142 @@
+"""Benchmark: Klein-Gordon grid-point updates/s on one MI355X.
+
+Workload (the full single-GPU configuration): massive (m = 0.1) Klein-Gordon
+field on Schwarzschild in Kerr-Schild coordinates (M = 1), Nr = 2048 on
+r in [1.8, 200], Ntheta = 128, Nphi = 256 (67.1 M points x 5 fields), classic
+RK4 with the reference's 2nd-order stencils, dt from CFL 0.25.  Initial phi
+is a Gaussian shell (r0 = 30, width 3) times sum_{l<=4} c_lm Y_lm with
+c_lm ~ N(0, 1e-3) from a seed; psi = its gradient, Pi = 0.
+
+value = Nr*Ntheta*Nphi * steps / wall time of the timed steps (state resident
+in HBM).  The timed steps run with a HIP event pair around every kernel on
+the engine's stream, which gives the average duration of the RHS+stage
+kernel for the roofline line.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_PER_UPDATE = {"rk4": 680, "rk3": 320, "pirk2": 248, "pirk3": 416}
+STAGES = {"rk4": 4, "rk3": 3, "pirk2": 3, "pirk3": 5}
+
+CFG = dict(Nx=2048, Ny=129, Nz=257, xmin=1.8, xmax=200.0, m=0.1, M_bh=1.0, a=0.0, CFL=0.25, scheme="rk4")
+
+
+def cpu_baseline(max_seconds=20.0):
+    """The oracle (C restatement, 1 thread) on a bounded sample of the workload:
+    the same radial/polar grid (Nr = 2048, Ntheta = 128) with 8 azimuthal planes,
+    RK4 steps until ~max_seconds of CPU time."""
+    import numpy as np
+    import oracle
+    from kg_amd.initial_data import shell_state, random_lm_coeffs
+
+    Nx, Ny, Nz = CFG["Nx"], CFG["Ny"], 9
+    sim = oracle.Sim(Nx, Ny, Nz, 2, CFG["xmin"], CFG["xmax"], "Kerr_Schild", 0.0, 1.0, CFG["m"], "outflow",
+                     CFG["CFL"])
+    sim.u[...] = shell_state(Nx, Ny, Nz, 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0, random_lm_coeffs(20261203))
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        sim.step("rk4", 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= max_seconds or steps >= 64:
+            break
+    ups = Nx * (Ny - 1) * (Nz - 1) * steps / el
+    return {"value": ups, "unit": "grid-point updates/s", "cores": 1, "kind": "port",
+            "sample": "oracle/kg_oracle.c RK4, %d steps on Nr=%d, Ntheta=%d, Nphi=%d (the workload's r-theta grid, "
+                      "8 phi planes), 1 thread, %.1f s" % (steps, Nx, Ny - 1, Nz - 1, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (a single grid lives on one MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import kg_amd
+    from kg_amd.initial_data import config2_state
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    print("bench: building initial data", file=sys.stderr, flush=True)
+    u0 = config2_state()
+    lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
+                            CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"])
+    lm.set_u_soa(u0[None])
+    del u0
+    print("bench: warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
+    lm.step(args.warmup)
+    lm.sync()
+    torch.cuda.synchronize()
+
+    lm.profile(True)
+    torch.cuda.synchronize()
+    lm.sync()
+    t0 = time.perf_counter()
+    lm.step(args.steps)
+    lm.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stage_ms, stage_n = lm.profile_read(0)
+    fill_ms, fill_n = lm.profile_read(1)
+    lm.profile(False)
+
+    u = lm.u_soa()[0]
+    finite = bool(np.isfinite(u).all())
+    lm.close()
+
+    npts = CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1)
+    elapsed = t1 - t0
+    value = npts * args.steps / elapsed
+    sch = CFG["scheme"]
+    per_launch_bytes = npts * B_PER_UPDATE[sch] / STAGES[sch]
+    avg_stage_s = stage_ms / 1e3 / max(stage_n, 1)
+    achieved = per_launch_bytes / avg_stage_s / 1e9
+    out = {
+        "metric": "KG grid-point updates/s (batched vs plain, <=1e-12 vs FP64 CPU) + p99 latency",
+        "value": value,
+        "unit": "grid-point updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian shell x sum of Y_lm, l<=4)",
+        "config": {"workload": "plain single-grid RK4, Nr=2048 r in [1.8,200], Ntheta=128, Nphi=256, m=0.1, "
+                               "Kerr-Schild M=1, CFL 0.25",
+                   "grid_points": npts, "scheme": "rk4", "fields": 5, "state_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_kernel (RHS + RK4 stage combine)",
+                     "avg_kernel_ms": avg_stage_s * 1e3, "launches": stage_n,
+                     "bytes_per_launch": per_launch_bytes},
+        "fill_ms_per_step": fill_ms / args.steps,
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

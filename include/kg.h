@@ -1,0 +1,141 @@
+/*
+ * kg.h -- C ABI of the MI355X engine for the Klein-Gordon method-of-lines
+ * step of FlavioRosales/KG (library: kg_amd/libkg_hip.so).
+ *
+ * The reference path sits behind the Fortran type-bound interface of
+ * `type(lines_method)` (MoL/lines_method_lib.f90:19-44): the driver calls
+ * `MoL%pirk2()` once per step (main/evolve.f90:25) on the state
+ * `MoL%u(nvars, iL-g:iR+g, jL-g:jR+g, kL:kR)`, complex(8), column-major.
+ * These entry points replace that interface one for one; a Fortran driver
+ * binds them through ISO_C_BINDING (kg_amd/fortran/kg_hip_mod.f90).
+ *
+ * Conventions
+ *  - Every function returns a kg_status; KG_OK = 0.  On error the message is
+ *    available from kg_last_error(h) (or kg_last_error(NULL) when no handle
+ *    exists yet).  The reference prints and stops on errors
+ *    (lines_method_lib.f90:529-533,744-748); the Fortran shim keeps that by
+ *    stopping on a nonzero status.
+ *  - One handle = one grid (the reference's one-rank object).  Calls on one
+ *    handle must come from one thread at a time.  kg_step and the kg_<scheme>
+ *    calls are asynchronous on the handle's HIP stream; downloads and
+ *    kg_sync synchronise.
+ *  - Host arrays are owned by the caller and only read or written inside
+ *    the call.  Device memory is owned by the handle.
+ *
+ * Reference sizes: Nx radial cells, Ny = Ntheta+1, Nz = Nphi+1, g = g_pts.
+ * Padded extents: ni = Nx+2g+1, nj = Ny+2g+1, nk = Nz+1
+ * (lines_method_lib.f90:135-138 with nproc = 1).
+ */
+#ifndef KG_H
+#define KG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t kg_status;
+enum {
+  KG_OK = 0,
+  KG_ERR_INVALID = 1,      /* bad argument or descriptor */
+  KG_ERR_UNSUPPORTED = 2,  /* a branch the reference stops on */
+  KG_ERR_HIP = 3,          /* HIP runtime error */
+  KG_ERR_NOMEM = 4,        /* device allocation failed */
+  KG_ERR_NO_DEVICE = 5     /* no GPU visible */
+};
+
+/* time integrators: lines_method_lib.f90 pirk2 :271-332, pirk3 :334-408,
+ * rk3 :211-269, rk4 :411-474 */
+enum { KG_PIRK2 = 0, KG_PIRK3 = 1, KG_RK3 = 2, KG_RK4 = 3 };
+/* metrics: geometry.f90:104-247.  The RHS of Pi exists only for Kerr_Schild
+ * and Schw_EF (lines_method_lib.f90:727-748); the others are rejected at
+ * kg_create with KG_ERR_UNSUPPORTED, as the reference stops there. */
+enum { KG_METRIC_MINKOWSKI = 0, KG_METRIC_KERR_SCHILD = 1, KG_METRIC_MKS = 2, KG_METRIC_SCHW_EF = 3 };
+/* boundary_type_rmax: 'outflow' / 'inflow' (lines_method_lib.f90:522-535) */
+enum { KG_BC_OUTFLOW = 0, KG_BC_INFLOW = 1 };
+/* host array layouts accepted by upload/download */
+enum {
+  KG_LAYOUT_FORTRAN = 0, /* u(nvars,ni,nj,nk), variable fastest: complex(8) pairs
+                            when is_complex, else real(8) -- MoL%u as is */
+  KG_LAYOUT_SOA = 1      /* [part][nvars][nk][nj][ni] real(8), r fastest; part =
+                            real, imaginary (imaginary only when is_complex) */
+};
+
+/* Descriptor: the arguments of lines_method_constructor
+ * (lines_method_lib.f90:49-51) plus the scheme and the value type. */
+typedef struct kg_desc {
+  int32_t nvars;      /* must be 5: phi, psi_r, psi_theta, psi_phi, Pi */
+  int32_t g_pts;      /* ghost points in r and theta (reference: 2) */
+  int32_t Nx, Ny, Nz; /* reference sizes (Ny = Ntheta+1, Nz = Nphi+1) */
+  int32_t scheme;     /* KG_PIRK2 ... KG_RK4: what kg_step advances with */
+  int32_t metric;     /* KG_METRIC_* */
+  int32_t bc_rmax;    /* KG_BC_* */
+  int32_t is_complex; /* 1: complex(8) state (two real systems on the device) */
+  int32_t reserved;
+  double xmin, xmax;  /* radial domain */
+  double tmin;        /* initial time */
+  double CFL;         /* used when dt <= 0 */
+  double dt;          /* > 0 overrides dt = CFL*min(...) (diagnostic_lib.f90:101-104) */
+  double a, M_bh;     /* black-hole spin and mass */
+  double m;           /* field mass */
+} kg_desc;
+
+typedef struct kg_handle kg_handle;
+
+/* lines_method_constructor (lines_method_lib.f90:49-129): grid, metric
+ * tables, dt; state and stage buffers zeroed. */
+kg_status kg_create(const kg_desc* desc, kg_handle** out);
+/* releases device memory (no reference counterpart: Fortran deallocation) */
+kg_status kg_destroy(kg_handle* h);
+/* message of the last failed call on h (h = NULL: last failure without a handle) */
+const char* kg_last_error(const kg_handle* h);
+
+/* dims[0..7] = Nx, Ny, Nz, g, ni, nj, nk, npts(padded) */
+kg_status kg_get_dims(const kg_handle* h, int64_t dims[8]);
+/* this%t and this%dt (lines_method_lib.f90:24,112) */
+kg_status kg_get_time(const kg_handle* h, double* t, double* dt);
+kg_status kg_set_time(kg_handle* h, double t);
+
+/* MoL%u <-> device (whole padded array, ghosts included) */
+kg_status kg_upload_state(kg_handle* h, const double* u, int32_t layout);
+kg_status kg_download_state(kg_handle* h, double* u, int32_t layout);
+/* The persistent PIRK stage arrays MoL%u_stage (which = 0) and
+ * MoL%u_stage_2 (which = 1).  Only their Pi component carries over between
+ * steps (lines_method_lib.f90:288-289,366-367); the full array is exchanged
+ * so a run can be resumed exactly. */
+kg_status kg_upload_aux(kg_handle* h, int32_t which, const double* u, int32_t layout);
+kg_status kg_download_aux(kg_handle* h, int32_t which, double* u, int32_t layout);
+/* Device-to-device state exchange in KG_LAYOUT_SOA order (for callers that
+ * keep data in HBM).  dir = 0: src -> handle; dir = 1: handle -> dst. */
+kg_status kg_copy_state_device(kg_handle* h, void* dev_ptr, int32_t dir);
+
+/* boundary() of the integrators (lines_method_lib.f90:313-330): xL, xR, yL,
+ * yR, zL, zR on MoL%u, as the initial-data code calls them
+ * (initial_conditions.f90:111-121). */
+kg_status kg_fill_ghosts(kg_handle* h);
+
+/* One step of a given scheme: MoL%pirk2() (lines_method_lib.f90:271),
+ * MoL%pirk3() (:334), MoL%rk3(rhs) (:211), MoL%rk4(rhs) (:411).  The rk3/rk4
+ * rhs callback is the Klein-Gordon right-hand side (all five rhs_* on u). */
+kg_status kg_pirk2(kg_handle* h);
+kg_status kg_pirk3(kg_handle* h);
+kg_status kg_rk3(kg_handle* h);
+kg_status kg_rk4(kg_handle* h);
+/* nsteps steps of desc.scheme: the `do l=1,Nt` loop body of evolve.f90:22-26
+ * without diagnostics, replayed from a captured HIP graph. */
+kg_status kg_step(kg_handle* h, int32_t nsteps);
+/* waits for the handle's stream */
+kg_status kg_sync(kg_handle* h);
+
+/* Kernel timing with HIP events on the handle's stream (no reference
+ * counterpart; the reference only prints MPI_Wtime, save_gnuplot.f90:154).
+ * kind: 0 = RHS+stage kernels, 1 = ghost-fill kernels.  While enabled,
+ * kg_step launches eagerly with an event pair around every kernel. */
+kg_status kg_profile_enable(kg_handle* h, int32_t on);
+kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KG_H */

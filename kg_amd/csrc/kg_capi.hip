@@ -1,0 +1,607 @@
+// kg_capi.hip -- the C ABI of include/kg.h: handles, device buffers, the
+// integrator stage sequences and their HIP-graph replay.
+//
+// Stage sequences (MoL/lines_method_lib.f90):
+//   rk4  (:411-474): 4 x [stage kernel (RHS + combine), x/y/z ghost fill]
+//   rk3  (:211-269): 3 x [stage kernel, fill]
+//   pirk2(:271-332): psi part, Pi part (reads last step's u_stage(5)),
+//                    final combine, fill of u
+//   pirk3(:334-408): as pirk2 with the second stage state u_stage_2
+// The fills that pirk2/pirk3 apply to u after stages 1 and 2 (:291, :353,
+// :369) have no effect on the result: u is overwritten from u_p in the last
+// stage and u_p was copied before them, so they are not issued.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/kg.h"
+#include "kg_internal.h"
+
+using namespace kg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct State5 {
+  double* f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+}  // namespace
+
+struct kg_handle {
+  kg_desc desc{};
+  HostGrid hg{};
+  Geom G{};
+  int nsys = 1;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* d_tab = nullptr;
+  State5 u;             // MoL%u (== u_p inside a step; updated in place)
+  State5 S;             // MoL%u_stage; S.f[4] is one of the two Pi slots
+  double* S5[2] = {nullptr, nullptr};
+  State5 S2;            // MoL%u_stage_2
+  double* S25[2] = {nullptr, nullptr};
+  State5 Ua, Ub, A;     // RK stage states and the RK4 accumulator
+  int p5 = 0, p25 = 0;  // slot holding the current u_stage(5), u_stage_2(5)
+  double t = 0.0;
+  std::string err;
+  std::vector<void*> allocs;
+  std::map<int, hipGraphExec_t> graphs;
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_marks;  // (kind, index of start event)
+  double prof_ms[2] = {0.0, 0.0};
+  int64_t prof_n[2] = {0, 0};
+};
+
+namespace {
+
+kg_status fail(kg_handle* h, kg_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  g_last_error = buf;
+  return st;
+}
+
+#define KG_HIP(h, expr)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((h), KG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+kg_status alloc_state(kg_handle* h, State5* s, int nfields_extra, double** extra) {
+  const size_t per = (size_t)h->G.sys_stride * h->nsys;  // doubles per field buffer
+  const size_t n = per * (5 + nfields_extra);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(double));
+  if (e != hipSuccess) return fail(h, KG_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * 8, hipGetErrorString(e));
+  h->allocs.push_back(p);
+  KG_HIP(h, hipMemsetAsync(p, 0, n * sizeof(double), h->stream));
+  double* d = (double*)p;
+  for (int c = 0; c < 5; ++c) s->f[c] = d + c * per;
+  for (int x = 0; x < nfields_extra; ++x) extra[x] = d + (5 + x) * per;
+  return KG_OK;
+}
+
+kg_status ensure_pirk(kg_handle* h, bool need_s2) {
+  if (!h->S.f[0]) {
+    double* ex[1];
+    kg_status st = alloc_state(h, &h->S, 1, ex);
+    if (st) return st;
+    h->S5[0] = h->S.f[4];
+    h->S5[1] = ex[0];
+  }
+  if (need_s2 && !h->S2.f[0]) {
+    double* ex[1];
+    kg_status st = alloc_state(h, &h->S2, 1, ex);
+    if (st) return st;
+    h->S25[0] = h->S2.f[4];
+    h->S25[1] = ex[0];
+  }
+  return KG_OK;
+}
+
+kg_status ensure_rk(kg_handle* h, bool need_acc) {
+  if (!h->Ua.f[0]) {
+    kg_status st = alloc_state(h, &h->Ua, 0, nullptr);
+    if (st) return st;
+    st = alloc_state(h, &h->Ub, 0, nullptr);
+    if (st) return st;
+  }
+  if (need_acc && !h->A.f[0]) {
+    kg_status st = alloc_state(h, &h->A, 0, nullptr);
+    if (st) return st;
+  }
+  return KG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+hipEvent_t next_event(kg_handle* h) {
+  if (h->ev_used == h->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->ev_pool.push_back(e);
+  }
+  return h->ev_pool[h->ev_used++];
+}
+
+struct Box {
+  int i_lo, i_hi, j_lo, j_hi, k_lo, k_hi;
+};
+
+Box full_box(const Geom& G) { return Box{0, G.ni, 0, G.nj, 0, G.nk}; }
+// Points whose evolved value reaches the state after the ghost fill: the
+// fill overwrites i = -g, i = Nx+g, the theta-ghost rows and the plane
+// k = Nz from other points, so a stage followed by a fill only computes
+// i in [-g+1, Nx+g-1], j in [1, Ny-1], k in [0, Nz-1].
+Box filled_box(const Geom& G) { return Box{1, G.ni - 1, G.g + 1, G.g + G.Ny, 0, G.Nz}; }
+
+kg_status stage(kg_handle* h, int mode, const Box& b, const State5& in, const State5* base, const State5* aux,
+                const State5* out, const State5* out2) {
+  StageArgs a{};
+  a.G = h->G;
+  a.tab = h->d_tab;
+  for (int c = 0; c < 5; ++c) {
+    a.in[c] = in.f[c];
+    a.base[c] = base ? base->f[c] : nullptr;
+    a.aux[c] = aux ? aux->f[c] : nullptr;
+    a.out[c] = out ? out->f[c] : nullptr;
+    a.out2[c] = out2 ? out2->f[c] : nullptr;
+  }
+  a.i_lo = b.i_lo;
+  a.i_hi = b.i_hi;
+  a.j_lo = b.j_lo;
+  a.j_hi = b.j_hi;
+  a.k_lo = b.k_lo;
+  a.k_hi = b.k_hi;
+  a.kc = 8;
+  a.nsys = h->nsys;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->prof) {
+    e0 = next_event(h);
+    e1 = next_event(h);
+    if (!e0 || !e1) return fail(h, KG_ERR_HIP, "hipEventCreate failed");
+    KG_HIP(h, hipEventRecord(e0, h->stream));
+  }
+  KG_HIP(h, launch_stage(mode, a, h->stream));
+  if (h->prof) {
+    KG_HIP(h, hipEventRecord(e1, h->stream));
+    h->ev_marks.push_back({0, h->ev_used - 2});
+  }
+  return KG_OK;
+}
+
+kg_status fill(kg_handle* h, const State5& u, const State5& u_p) {
+  FillArgs f{};
+  f.G = h->G;
+  for (int c = 0; c < 5; ++c) {
+    f.u[c] = u.f[c];
+    f.u_p[c] = u_p.f[c];
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->prof) {
+    e0 = next_event(h);
+    e1 = next_event(h);
+    if (!e0 || !e1) return fail(h, KG_ERR_HIP, "hipEventCreate failed");
+    KG_HIP(h, hipEventRecord(e0, h->stream));
+  }
+  for (int ph = 0; ph < 3; ++ph) KG_HIP(h, launch_fill(f, h->stream, ph, h->nsys));
+  if (h->prof) {
+    KG_HIP(h, hipEventRecord(e1, h->stream));
+    h->ev_marks.push_back({1, h->ev_used - 2});
+  }
+  return KG_OK;
+}
+
+#define KG_TRY(expr)              \
+  do {                            \
+    kg_status s_ = (expr);        \
+    if (s_ != KG_OK) return s_;   \
+  } while (0)
+
+kg_status enqueue_rk4(kg_handle* h) {
+  const Geom& G = h->G;
+  const Box fb = filled_box(G);
+  KG_TRY(stage(h, M_RK4_S1, fb, h->u, &h->u, nullptr, &h->Ua, &h->A));
+  KG_TRY(fill(h, h->Ua, h->u));
+  KG_TRY(stage(h, M_RK4_S2, fb, h->Ua, &h->u, &h->A, &h->Ub, &h->A));
+  KG_TRY(fill(h, h->Ub, h->u));
+  KG_TRY(stage(h, M_RK4_S3, fb, h->Ub, &h->u, &h->A, &h->Ua, &h->A));
+  KG_TRY(fill(h, h->Ua, h->u));
+  KG_TRY(stage(h, M_RK4_S4, fb, h->Ua, &h->u, &h->A, &h->u, nullptr));
+  KG_TRY(fill(h, h->u, h->u));
+  return KG_OK;
+}
+
+kg_status enqueue_rk3(kg_handle* h) {
+  const Box fb = filled_box(h->G);
+  KG_TRY(stage(h, M_RK3_S1, fb, h->u, &h->u, nullptr, &h->Ua, nullptr));
+  KG_TRY(fill(h, h->Ua, h->u));
+  KG_TRY(stage(h, M_RK3_S2, fb, h->Ua, &h->u, nullptr, &h->Ub, nullptr));
+  KG_TRY(fill(h, h->Ub, h->u));
+  KG_TRY(stage(h, M_RK3_S3, fb, h->Ub, &h->u, nullptr, &h->u, nullptr));
+  KG_TRY(fill(h, h->u, h->u));
+  return KG_OK;
+}
+
+// PIRK stage 1 (:278-289): S(1..4) from u, then S(5) from S(1..4) and the
+// previous step's S(5) held in slot p5.  Leaves the new S(5) in slot 1-p5.
+kg_status enqueue_pirk_stage1(kg_handle* h, int p5) {
+  const Box all = full_box(h->G);
+  State5 s_out = h->S;
+  KG_TRY(stage(h, M_PIRK_1A, all, h->u, &h->u, nullptr, &s_out, nullptr));
+  State5 in = h->S;
+  in.f[4] = h->S5[p5];
+  State5 out{};
+  out.f[4] = h->S5[1 - p5];
+  KG_TRY(stage(h, M_PIRK_1B, all, in, &h->u, nullptr, &out, nullptr));
+  return KG_OK;
+}
+
+kg_status enqueue_pirk2(kg_handle* h, int p5) {
+  KG_TRY(enqueue_pirk_stage1(h, p5));
+  State5 s = h->S;
+  s.f[4] = h->S5[1 - p5];
+  // :294-304 u = 0.5*(u_p + u_stage + dt*rhs(u_stage)), then boundary (:306)
+  KG_TRY(stage(h, M_PIRK2_2, filled_box(h->G), s, &h->u, nullptr, &h->u, nullptr));
+  KG_TRY(fill(h, h->u, h->u));
+  return KG_OK;
+}
+
+kg_status enqueue_pirk3(kg_handle* h, int p5, int p25) {
+  KG_TRY(enqueue_pirk_stage1(h, p5));
+  State5 s = h->S;
+  s.f[4] = h->S5[1 - p5];
+  const Box all = full_box(h->G);
+  // :356-364 u_stage_2(1..4)
+  State5 s2out = h->S2;
+  KG_TRY(stage(h, M_PIRK3_2A, all, s, &h->u, &s, &s2out, nullptr));
+  // :366-367 u_stage_2(5) from rhs_pi(u_stage_2) with the previous u_stage_2(5)
+  State5 in2 = h->S2;
+  in2.f[4] = h->S25[p25];
+  State5 out2{};
+  out2.f[4] = h->S25[1 - p25];
+  KG_TRY(stage(h, M_PIRK3_2B, all, in2, &h->u, &s, &out2, nullptr));
+  // :372-384
+  State5 s2 = h->S2;
+  s2.f[4] = h->S25[1 - p25];
+  KG_TRY(stage(h, M_PIRK3_3, filled_box(h->G), s2, &h->u, nullptr, &h->u, nullptr));
+  KG_TRY(fill(h, h->u, h->u));
+  return KG_OK;
+}
+
+kg_status enqueue_step(kg_handle* h, int scheme) {
+  switch (scheme) {
+    case KG_RK4: return enqueue_rk4(h);
+    case KG_RK3: return enqueue_rk3(h);
+    case KG_PIRK2: return enqueue_pirk2(h, h->p5);
+    case KG_PIRK3: return enqueue_pirk3(h, h->p5, h->p25);
+  }
+  return fail(h, KG_ERR_INVALID, "unknown scheme %d", scheme);
+}
+
+void advance_parity(kg_handle* h, int scheme) {
+  if (scheme == KG_PIRK2 || scheme == KG_PIRK3) h->p5 ^= 1;
+  if (scheme == KG_PIRK3) h->p25 ^= 1;
+}
+
+kg_status ensure_buffers(kg_handle* h, int scheme) {
+  if (scheme == KG_RK4) return ensure_rk(h, true);
+  if (scheme == KG_RK3) return ensure_rk(h, false);
+  if (scheme == KG_PIRK2) return ensure_pirk(h, false);
+  if (scheme == KG_PIRK3) return ensure_pirk(h, true);
+  return fail(h, KG_ERR_INVALID, "unknown scheme %d", scheme);
+}
+
+kg_status do_steps(kg_handle* h, int scheme, int nsteps) {
+  if (nsteps < 0) return fail(h, KG_ERR_INVALID, "nsteps < 0");
+  KG_TRY(ensure_buffers(h, scheme));
+  for (int it = 0; it < nsteps; ++it) {
+    if (h->prof) {
+      KG_TRY(enqueue_step(h, scheme));
+    } else {
+      const int key = scheme * 4 + h->p5 * 2 + h->p25;
+      auto itg = h->graphs.find(key);
+      if (itg == h->graphs.end()) {
+        hipGraph_t graph;
+        KG_HIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        kg_status st = enqueue_step(h, scheme);
+        hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+        if (st != KG_OK) return st;
+        if (ec != hipSuccess) return fail(h, KG_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        hipGraphExec_t exec;
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(h, KG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        itg = h->graphs.emplace(key, exec).first;
+      }
+      KG_HIP(h, hipGraphLaunch(itg->second, h->stream));
+    }
+    advance_parity(h, scheme);
+    h->t = h->t + h->desc.dt;  // this%t = this%t + this%dt
+  }
+  return KG_OK;
+}
+
+// host layout conversion: Fortran u(5,ni,nj,nk) [complex pairs] <-> dense [sys][5][nk][nj][ni]
+void fortran_to_dense(const kg_handle* h, const double* src, double* dst) {
+  const Geom& G = h->G;
+  const int64_t npts = (int64_t)G.ni * G.nj * G.nk;
+  const int ns = h->nsys;
+  for (int64_t p = 0; p < npts; ++p)
+    for (int c = 0; c < 5; ++c)
+      for (int s = 0; s < ns; ++s) dst[((int64_t)s * 5 + c) * npts + p] = src[(p * 5 + c) * ns + s];
+}
+
+void dense_to_fortran(const kg_handle* h, const double* src, double* dst) {
+  const Geom& G = h->G;
+  const int64_t npts = (int64_t)G.ni * G.nj * G.nk;
+  const int ns = h->nsys;
+  for (int64_t p = 0; p < npts; ++p)
+    for (int c = 0; c < 5; ++c)
+      for (int s = 0; s < ns; ++s) dst[(p * 5 + c) * ns + s] = src[((int64_t)s * 5 + c) * npts + p];
+}
+
+kg_status xfer(kg_handle* h, const State5& st, double* host, int32_t layout, int to_dev) {
+  if (!host) return fail(h, KG_ERR_INVALID, "null host pointer");
+  if (layout != KG_LAYOUT_FORTRAN && layout != KG_LAYOUT_SOA) return fail(h, KG_ERR_INVALID, "bad layout %d", layout);
+  const Geom& G = h->G;
+  const size_t n = (size_t)G.ni * G.nj * G.nk * 5 * h->nsys;
+  std::vector<double> tmp;
+  double* dense_host = host;
+  if (layout == KG_LAYOUT_FORTRAN) {
+    tmp.resize(n);
+    dense_host = tmp.data();
+    if (to_dev) fortran_to_dense(h, host, dense_host);
+  }
+  void* dbuf = nullptr;
+  KG_HIP(h, hipMalloc(&dbuf, n * sizeof(double)));
+  kg_status rc = KG_OK;
+  hipError_t e = hipSuccess;
+  if (to_dev) {
+    e = hipMemcpyAsync(dbuf, dense_host, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_pack(G, (double*)dbuf, st.f, h->nsys, 1, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  } else {
+    e = launch_pack(G, (double*)dbuf, st.f, h->nsys, 0, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dense_host, dbuf, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  }
+  if (e != hipSuccess) rc = fail(h, KG_ERR_HIP, "state transfer failed: %s", hipGetErrorString(e));
+  (void)hipFree(dbuf);
+  if (rc == KG_OK && !to_dev && layout == KG_LAYOUT_FORTRAN) dense_to_fortran(h, dense_host, host);
+  return rc;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* kg_last_error(const kg_handle* h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+kg_status kg_create(const kg_desc* d, kg_handle** out) {
+  if (!d || !out) return fail(nullptr, KG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (d->nvars != 5) return fail(nullptr, KG_ERR_INVALID, "nvars must be 5 (got %d)", d->nvars);
+  if (d->g_pts < 1 || d->g_pts > 8) return fail(nullptr, KG_ERR_INVALID, "g_pts out of range");
+  if (d->Nx < 4) return fail(nullptr, KG_ERR_INVALID, "Nx must be >= 4");
+  if (d->Ny < d->g_pts + 2) return fail(nullptr, KG_ERR_INVALID, "Ny must be >= g_pts+2");
+  if (d->Nz < 4) return fail(nullptr, KG_ERR_INVALID, "Nz must be >= 4");
+  if (d->scheme < KG_PIRK2 || d->scheme > KG_RK4) return fail(nullptr, KG_ERR_INVALID, "bad scheme");
+  if (d->metric != KG_METRIC_KERR_SCHILD && d->metric != KG_METRIC_SCHW_EF)
+    return fail(nullptr, KG_ERR_UNSUPPORTED, "Metric %d not implemented yet (rhs_pi supports Kerr_Schild, Schw_EF)",
+                d->metric);
+  if (d->bc_rmax != KG_BC_OUTFLOW && d->bc_rmax != KG_BC_INFLOW)
+    return fail(nullptr, KG_ERR_UNSUPPORTED, "This boundary condition type for r_max is not implemented yet");
+  if (!(d->xmax > d->xmin)) return fail(nullptr, KG_ERR_INVALID, "xmax must exceed xmin");
+  if (!(d->dt > 0.0) && !(d->CFL > 0.0)) return fail(nullptr, KG_ERR_INVALID, "need dt > 0 or CFL > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, KG_ERR_NO_DEVICE, "no GPU visible");
+
+  kg_handle* h = new kg_handle();
+  h->desc = *d;
+  h->nsys = d->is_complex ? 2 : 1;
+  build_grid(d->Nx, d->Ny, d->Nz, d->g_pts, d->xmin, d->xmax, d->CFL, d->dt, &h->hg);
+  h->desc.dt = h->hg.dt;
+  h->t = d->tmin;
+  Geom& G = h->G;
+  G.ni = h->hg.ni;
+  G.nj = h->hg.nj;
+  G.nk = h->hg.nk;
+  G.ld = h->hg.ld;
+  G.Nx = d->Nx;
+  G.Ny = d->Ny;
+  G.Nz = d->Nz;
+  G.g = d->g_pts;
+  G.plane = (int64_t)G.ld * G.nj;
+  G.fstride = (G.plane * G.nk + 31) / 32 * 32;
+  G.sys_stride = G.fstride;
+  G.rr = 0.50 / h->hg.dx;
+  G.rt = 0.50 / h->hg.dy;
+  G.rp = 0.50 / h->hg.dz;
+  G.dt = h->hg.dt;
+  G.metric_ef = d->metric == KG_METRIC_SCHW_EF;
+  G.bc_inflow = d->bc_rmax == KG_BC_INFLOW;
+
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreate(&h->stream);
+  if (e != hipSuccess) {
+    fail(nullptr, KG_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    delete h;
+    return KG_ERR_HIP;
+  }
+  std::vector<double> tab((size_t)kNumTables * G.nj * G.ld);
+  build_tables(h->hg, d->metric, d->a, d->M_bh, d->m, tab.data());
+  e = hipMalloc(&h->d_tab, tab.size() * sizeof(double));
+  if (e == hipSuccess) {
+    h->allocs.push_back(h->d_tab);
+    e = hipMemcpy(h->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    fail(nullptr, KG_ERR_NOMEM, "metric table upload failed: %s", hipGetErrorString(e));
+    kg_destroy(h);
+    return KG_ERR_NOMEM;
+  }
+  kg_status st = alloc_state(h, &h->u, 0, nullptr);
+  if (st == KG_OK) st = ensure_buffers(h, d->scheme);
+  if (st == KG_OK && hipStreamSynchronize(h->stream) != hipSuccess) st = KG_ERR_HIP;
+  if (st != KG_OK) {
+    g_last_error = h->err;
+    kg_destroy(h);
+    return st;
+  }
+  *out = h;
+  return KG_OK;
+}
+
+kg_status kg_destroy(kg_handle* h) {
+  if (!h) return KG_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return KG_OK;
+}
+
+kg_status kg_get_dims(const kg_handle* h, int64_t dims[8]) {
+  if (!h || !dims) return fail(nullptr, KG_ERR_INVALID, "null argument");
+  const Geom& G = h->G;
+  dims[0] = G.Nx;
+  dims[1] = G.Ny;
+  dims[2] = G.Nz;
+  dims[3] = G.g;
+  dims[4] = G.ni;
+  dims[5] = G.nj;
+  dims[6] = G.nk;
+  dims[7] = (int64_t)G.ni * G.nj * G.nk;
+  return KG_OK;
+}
+
+kg_status kg_get_time(const kg_handle* h, double* t, double* dt) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  if (t) *t = h->t;
+  if (dt) *dt = h->desc.dt;
+  return KG_OK;
+}
+
+kg_status kg_set_time(kg_handle* h, double t) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  h->t = t;
+  return KG_OK;
+}
+
+kg_status kg_upload_state(kg_handle* h, const double* u, int32_t layout) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  return xfer(h, h->u, const_cast<double*>(u), layout, 1);
+}
+
+kg_status kg_download_state(kg_handle* h, double* u, int32_t layout) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  return xfer(h, h->u, u, layout, 0);
+}
+
+static kg_status aux_state(kg_handle* h, int32_t which, State5* s) {
+  if (which == 0) {
+    KG_TRY(ensure_pirk(h, false));
+    *s = h->S;
+    s->f[4] = h->S5[h->p5];
+  } else if (which == 1) {
+    KG_TRY(ensure_pirk(h, true));
+    *s = h->S2;
+    s->f[4] = h->S25[h->p25];
+  } else {
+    return fail(h, KG_ERR_INVALID, "aux index must be 0 (u_stage) or 1 (u_stage_2)");
+  }
+  return KG_OK;
+}
+
+kg_status kg_upload_aux(kg_handle* h, int32_t which, const double* u, int32_t layout) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  State5 s;
+  KG_TRY(aux_state(h, which, &s));
+  return xfer(h, s, const_cast<double*>(u), layout, 1);
+}
+
+kg_status kg_download_aux(kg_handle* h, int32_t which, double* u, int32_t layout) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  State5 s;
+  KG_TRY(aux_state(h, which, &s));
+  return xfer(h, s, u, layout, 0);
+}
+
+kg_status kg_copy_state_device(kg_handle* h, void* dev_ptr, int32_t dir) {
+  if (!h || !dev_ptr) return fail(h, KG_ERR_INVALID, "null argument");
+  KG_HIP(h, launch_pack(h->G, (double*)dev_ptr, h->u.f, h->nsys, dir == 0 ? 1 : 0, h->stream));
+  KG_HIP(h, hipStreamSynchronize(h->stream));
+  return KG_OK;
+}
+
+kg_status kg_fill_ghosts(kg_handle* h) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  return fill(h, h->u, h->u);
+}
+
+kg_status kg_pirk2(kg_handle* h) { return h ? do_steps(h, KG_PIRK2, 1) : fail(nullptr, KG_ERR_INVALID, "null handle"); }
+kg_status kg_pirk3(kg_handle* h) { return h ? do_steps(h, KG_PIRK3, 1) : fail(nullptr, KG_ERR_INVALID, "null handle"); }
+kg_status kg_rk3(kg_handle* h) { return h ? do_steps(h, KG_RK3, 1) : fail(nullptr, KG_ERR_INVALID, "null handle"); }
+kg_status kg_rk4(kg_handle* h) { return h ? do_steps(h, KG_RK4, 1) : fail(nullptr, KG_ERR_INVALID, "null handle"); }
+
+kg_status kg_step(kg_handle* h, int32_t nsteps) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  return do_steps(h, h->desc.scheme, nsteps);
+}
+
+kg_status kg_sync(kg_handle* h) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  KG_HIP(h, hipStreamSynchronize(h->stream));
+  return KG_OK;
+}
+
+kg_status kg_profile_enable(kg_handle* h, int32_t on) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  KG_HIP(h, hipStreamSynchronize(h->stream));
+  h->prof = on != 0;
+  h->ev_used = 0;
+  h->ev_marks.clear();
+  h->prof_ms[0] = h->prof_ms[1] = 0.0;
+  h->prof_n[0] = h->prof_n[1] = 0;
+  return KG_OK;
+}
+
+kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t* launches) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  if (kind < 0 || kind > 1) return fail(h, KG_ERR_INVALID, "kind must be 0 or 1");
+  KG_HIP(h, hipStreamSynchronize(h->stream));
+  for (auto& m : h->ev_marks) {
+    float ms = 0.f;
+    KG_HIP(h, hipEventElapsedTime(&ms, h->ev_pool[m.second], h->ev_pool[m.second + 1]));
+    h->prof_ms[m.first] += ms;
+    h->prof_n[m.first] += 1;
+  }
+  h->ev_marks.clear();
+  h->ev_used = 0;
+  if (total_ms) *total_ms = h->prof_ms[kind];
+  if (launches) *launches = h->prof_n[kind];
+  return KG_OK;
+}
+
+}  // extern "C"

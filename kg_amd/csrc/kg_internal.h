@@ -1,0 +1,103 @@
+// kg_internal.h -- shared definitions of the HIP engine (host and device).
+//
+// Device layout of one real system: 5 fields, each [nk][nj][ld] doubles,
+// r fastest, ld = ni rounded up to 16 doubles (128-byte rows).  The padded
+// index ranges follow the reference: ii = i+g in [0,ni), jj = j+g in [0,nj),
+// kk = k in [0,nk) (MoL/lines_method_lib.f90:135-138).
+//
+// Metric coefficients are 2-D tables [nj][ld]: every metric of the reference
+// depends on (r, theta) only (MoL/geometry.f90:104-247).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kg {
+
+constexpr int kNumFields = 5;  // phi, psi_r, psi_theta, psi_phi, Pi
+constexpr int kNumTables = 10;
+
+// table slots
+enum Table : int {
+  T_ALPHA = 0,   // alpha
+  T_BETA = 1,    // beta^r
+  T_DXBETA = 2,  // d_r beta^r = Re dx1(beta)         (geometry.f90:344)
+  T_SQRTG = 3,   // sqrt(gamma)                       (geometry.f90:267)
+  T_ASG = 4,     // alpha*sqrtg (the reference's product, rounded once)
+  T_GRR = 5,     // gamma^{rr}
+  T_GRP = 6,     // gamma^{r phi}
+  T_ASGTT = 7,   // (alpha*sqrtg)*gamma^{theta theta}
+  T_GPP = 8,     // gamma^{phi phi}
+  T_AM2 = 9      // alpha*m**2
+};
+
+struct Geom {
+  int ni, nj, nk, ld;     // padded extents, row pitch (doubles)
+  int Nx, Ny, Nz, g;
+  int64_t plane;          // ld*nj
+  int64_t fstride;        // plane*nk rounded up: size of one field of one system
+  int64_t sys_stride;     // offset between systems inside a field buffer
+  double rr, rt, rp;      // 0.5/dx, 0.5/dy, 0.5/dz
+  double dt;
+  int metric_ef;          // 1: Schw_EF form of rhs_pi (lines_method_lib.f90:738-743)
+  int bc_inflow;          // 1: 'inflow' at r_max
+};
+
+// Host-side table builder (kg_metric.cpp): fills 10 tables [nj][ld] and the
+// grid spacings.  Returns 0 on success.
+struct HostGrid {
+  int Nx, Ny, Nz, g, ni, nj, nk, ld;
+  double xmin, xmax, dx, dy, dz, dt;
+};
+int build_grid(int Nx, int Ny, int Nz, int g, double xmin, double xmax, double CFL, double dt_override,
+               HostGrid* out);
+int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m, double* tables /*10*nj*ld*/);
+
+// Integrator stage kinds (one kernel launch each).
+enum StageMode : int {
+  M_RK4_S1 = 0,   // k = R(P)dt; A = k; U = P + 0.5k
+  M_RK4_S2 = 1,   // k = R(U)dt; A = A + 2k; U' = P + 0.5k
+  M_RK4_S3 = 2,   // k = R(U)dt; A = A + 2k; U' = P + k
+  M_RK4_S4 = 3,   // k = R(U)dt; P = P + (A + k)/6
+  M_RK3_S1 = 4,   // U1 = P + R(P)dt
+  M_RK3_S2 = 5,   // U2 = 0.75P + 0.25(U1 + R(U1)dt)
+  M_RK3_S3 = 6,   // P = P/3 + 2(U2 + R(U2)dt)/3
+  M_PIRK_1A = 7,  // S_c = u_c + dt R_c(u), c = 1..4
+  M_PIRK_1B = 8,  // S5' = u5 + dt R_pi(S1..4, S5)
+  M_PIRK2_2 = 9,  // u = 0.5((u + S) + dt R(S))
+  M_PIRK3_2A = 10,  // S2_c = 0.75u_c + 0.25(S_c + dt R_c(S)), c = 1..4
+  M_PIRK3_2B = 11,  // S2_5' = 0.75u5 + 0.25(S5 + dt R_pi(S2_1..4, S2_5))
+  M_PIRK3_3 = 12,   // u = (1/3)u + (2/3)(S2 + dt R(S2))
+  M_NUM_MODES = 13
+};
+
+struct StageArgs {
+  Geom G;
+  const double* tab;        // 10 tables, each nj*ld
+  const double* in[5];      // fields the RHS is evaluated on
+  const double* base[5];    // u_p / P / u
+  const double* aux[5];     // A (RK4), S (PIRK3 stage 2), S (PIRK2 stage 2)
+  double* out[5];           // stage output
+  double* out2[5];          // RK4 accumulator output
+  int i_lo, i_hi, j_lo, j_hi, k_lo, k_hi;  // computed box, [lo, hi)
+  int kc;                   // k-chunk per workgroup
+  int nti, ntj, ntk;        // tile counts (set by launch_stage)
+  int nsys;                 // independent real systems sharing the grid
+};
+
+struct FillArgs {
+  Geom G;
+  double* u[5];
+  const double* u_p[5];
+};
+
+// Device buffers hold fields as [field][system][nk][nj][ld]; a field pointer
+// addresses system 0 and system s sits sys_stride doubles further.
+
+// launchers (kg_kernels.hip)
+hipError_t launch_stage(int mode, const StageArgs& a, hipStream_t s);
+hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase /*0 x,1 y,2 z*/, int nsys);
+// dense [sys][5][nk][nj][ni] (device) <-> pitched fields; to_dev = 1: dense -> fields
+hipError_t launch_pack(const Geom& G, double* dense, double* const fields[5], int nsys, int to_dev, hipStream_t s);
+
+}  // namespace kg

@@ -1,0 +1,197 @@
+"""CPU FP64 oracle for the Klein-Gordon method-of-lines path (test infrastructure only).
+
+Loads ``oracle/libkg_oracle.so`` (the C restatement in ``kg_oracle.c``) and, when
+it has been built, ``oracle/_ref/libkgref_fd.so`` (the reference's own
+``tools/finite_differences.f90`` compiled from source with bind(C) wrappers).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg import this package, and only as the checker.  The product (``kg_amd``)
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libkg_oracle.so")
+_REF = os.path.join(_HERE, "_ref", "libkgref_fd.so")
+
+PIRK2, PIRK3, RK3, RK4 = 0, 1, 2, 3
+SCHEMES = {"pirk2": PIRK2, "pirk3": PIRK3, "rk3": RK3, "rk4": RK4}
+MINKOWSKI, KERR_SCHILD, MKS, SCHW_EF = 0, 1, 2, 3
+METRICS = {"Minkowski": MINKOWSKI, "Kerr_Schild": KERR_SCHILD, "MKS": MKS, "Schw_EF": SCHW_EF}
+OUTFLOW, INFLOW = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+def build(with_ref: bool | None = None) -> None:
+    """Compile the oracle (and, when /root/reference exists, the reference stencils)."""
+    if with_ref is None:
+        with_ref = os.path.isdir("/root/reference")
+    target = "ref" if with_ref else "all"
+    subprocess.run(["make", "-s", "-C", _HERE, target], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB):
+        raise RuntimeError("oracle/libkg_oracle.so is not built (run __graft_entry__.build())")
+    lib = C.CDLL(_LIB)
+    lib.kgo_create.restype = C.c_void_p
+    lib.kgo_create.argtypes = [C.c_int] * 4 + [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                               C.c_double, C.c_int, C.c_double, C.c_double]
+    lib.kgo_destroy.argtypes = [C.c_void_p]
+    lib.kgo_field.restype = _dp
+    lib.kgo_field.argtypes = [C.c_void_p, C.c_int]
+    lib.kgo_table.restype = _dp
+    lib.kgo_table.argtypes = [C.c_void_p, C.c_int]
+    lib.kgo_info.argtypes = [C.c_void_p, _i64p, _dp]
+    lib.kgo_reference_dt.restype = C.c_double
+    lib.kgo_reference_dt.argtypes = [C.c_int] * 4 + [C.c_double] * 3
+    for nm in ("kgo_dx1_raw", "kgo_dx2_raw", "kgo_dx3_raw"):
+        getattr(lib, nm).argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp]
+    lib.kgo_advec_x_raw.argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp, _dp]
+    for nm in ("kgo_rhs_phi", "kgo_rhs_psi1", "kgo_rhs_psi2", "kgo_rhs_psi3", "kgo_rhs_pi", "kgo_rhs_all"):
+        getattr(lib, nm).argtypes = [C.c_void_p, _dp, _dp]
+    lib.kgo_fill.argtypes = [C.c_void_p, _dp, _dp]
+    lib.kgo_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.kgo_fill_source_map.argtypes = [C.c_void_p, _i64p]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def reference_dt(Nx, Ny, Nz, g, xmin, xmax, CFL):
+    return lib().kgo_reference_dt(Nx, Ny, Nz, g, xmin, xmax, CFL)
+
+
+class Sim:
+    """One simulation of the oracle: padded real FP64 arrays [5][nk][nj][ni]."""
+
+    def __init__(self, Nx, Ny, Nz, g=2, xmin=1.8, xmax=60.0, metric="Kerr_Schild", a=0.0, M_bh=1.0,
+                 m=0.0, bc="outflow", CFL=0.25, dt=0.0):
+        L = lib()
+        self._h = L.kgo_create(Nx, Ny, Nz, g, xmin, xmax, METRICS[metric], a, M_bh, m,
+                               OUTFLOW if bc == "outflow" else INFLOW, CFL, dt)
+        if not self._h:
+            raise ValueError("invalid oracle configuration")
+        o = (C.c_int64 * 8)()
+        d = (C.c_double * 8)()
+        L.kgo_info(self._h, o, d)
+        self.Nx, self.Ny, self.Nz, self.g, self.ni, self.nj, self.nk, self.npts = [int(v) for v in o]
+        self.dx, self.dy, self.dz, self.dt = d[0], d[1], d[2], d[3]
+        self.shape = (5, self.nk, self.nj, self.ni)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kgo_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _view(self, which):
+        p = lib().kgo_field(self._h, which)
+        return np.ctypeslib.as_array(p, shape=self.shape)
+
+    @property
+    def u(self):
+        return self._view(0)
+
+    @property
+    def u_stage(self):
+        return self._view(2)
+
+    @property
+    def u_stage_2(self):
+        return self._view(3)
+
+    def table(self, which):
+        names = {"alpha": 0, "beta_r": 1, "dx_beta": 2, "sqrtg": 3, "g_rr": 4, "g_rp": 5, "g_tt": 6,
+                 "g_pp": 7}
+        if which in names:
+            p = lib().kgo_table(self._h, names[which])
+            return np.ctypeslib.as_array(p, shape=(self.nj, self.ni)).copy()
+        if which in ("g_dn", "g_up"):
+            p = lib().kgo_table(self._h, 8 if which == "g_dn" else 9)
+            return np.ctypeslib.as_array(p, shape=(4, 4, self.nj, self.ni)).copy()
+        n = {"x": self.ni, "y": self.nj, "z": self.nk}[which]
+        p = lib().kgo_table(self._h, {"x": 10, "y": 11, "z": 12}[which])
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def step(self, scheme, nsteps=1):
+        lib().kgo_step(self._h, SCHEMES[scheme] if isinstance(scheme, str) else scheme, nsteps)
+
+    def rhs_all(self, vars5):
+        out = np.zeros(self.shape)
+        v = np.ascontiguousarray(vars5, dtype=np.float64)
+        lib().kgo_rhs_all(self._h, _ptr(out), _ptr(v))
+        return out
+
+    def fill(self, u, u_p=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        up = np.ascontiguousarray(u if u_p is None else u_p, dtype=np.float64).copy()
+        lib().kgo_fill(self._h, _ptr(u), _ptr(up))
+        return u
+
+    def fill_source_map(self):
+        out = np.zeros(self.npts, dtype=np.int64)
+        lib().kgo_fill_source_map(self._h, out.ctypes.data_as(_i64p))
+        return out.reshape(self.nk, self.nj, self.ni)
+
+
+def stencil(name, f, h, beta=None):
+    """Oracle stencil on a real padded array f[nk][nj][ni]."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nk, nj, ni = f.shape
+    out = np.zeros_like(f)
+    L = lib()
+    if name == "advec_x":
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        L.kgo_advec_x_raw(ni, nj, nk, h, _ptr(f), _ptr(b), _ptr(out))
+    else:
+        getattr(L, "kgo_%s_raw" % name)(ni, nj, nk, h, _ptr(f), _ptr(out))
+    return out
+
+
+def have_reference() -> bool:
+    return os.path.exists(_REF)
+
+
+_ref = None
+
+
+def ref_stencil(name, f, h, g, beta=None):
+    """The reference's own stencil (complex(8) in, complex(8) out)."""
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(_REF)
+        for nm in ("kgref_dx1", "kgref_dx2", "kgref_dx3"):
+            getattr(_ref, nm).argtypes = [C.c_int] * 4 + [C.c_double, C.c_void_p, C.c_void_p]
+        _ref.kgref_advec_x.argtypes = [C.c_int] * 4 + [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    nk, nj, ni = f.shape
+    out = np.zeros_like(f)
+    if name == "advec_x":
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        _ref.kgref_advec_x(ni, nj, nk, g, h, f.ctypes.data, b.ctypes.data, out.ctypes.data)
+    else:
+        getattr(_ref, "kgref_" + name)(ni, nj, nk, g, h, f.ctypes.data, out.ctypes.data)
+    return out

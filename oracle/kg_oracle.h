@@ -1,0 +1,146 @@
+/*
+ * kg_oracle.h -- CPU FP64 restatement of the Klein-Gordon method-of-lines
+ * step of FlavioRosales/KG.  TEST INFRASTRUCTURE ONLY: this library is the
+ * checker for the HIP engine in kg_amd/.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product never does.
+ *
+ * It follows the reference's whole-array Fortran statements one by one
+ * (temporaries, operand order and left-to-right association kept), on real
+ * FP64 arrays.  A complex reference state is handled as two real systems
+ * (real and imaginary parts), which is exact because every coefficient on
+ * the path is real (MoL/lines_method_lib.f90:671-753).
+ *
+ * Array layout: dense padded 3-D arrays, r fastest:
+ *   a[(kk*nj + jj)*ni + ii],  ii = i+g in [0,ni), jj = j+g in [0,nj), kk = k in [0,nk)
+ * with ni = Nx+2g+1, nj = Ny+2g+1, nk = Nz+1 (MoL/lines_method_lib.f90:135-138).
+ * A 5-field state is 5 such arrays back to back (field 0 = phi, 1 = psi_r,
+ * 2 = psi_theta, 3 = psi_phi, 4 = Pi).
+ *
+ * Metric coefficients are stored as 2-D (ni x nj) tables: every metric the
+ * reference implements depends on x (r) and y (theta) only
+ * (MoL/geometry.f90:104-247), so the k index of the reference's 3-D arrays
+ * carries no information.
+ *
+ * Parity status per function (pins live in tests/):
+ *   kgo_dx1/kgo_dx2/kgo_dx3/kgo_advec_x: bit-exact against the reference's own
+ *     tools/finite_differences.f90 compiled from source (oracle/_ref), plus
+ *     polynomial exactness.
+ *   kgo_metric_build: closed-form Schwarzschild/Kerr-Schild identities and the
+ *     reference's own inverse-metric self-check (geometry.f90:409-492).
+ *   kgo_fill: index-tag maps checked against the closed-form reflection maps.
+ *   kgo_rk3/kgo_rk4/kgo_pirk2/kgo_pirk3: amplification polynomials on linear
+ *     scalar problems through the same stage code (kgo_*_generic), temporal
+ *     convergence order on the full operator, and exact symmetry invariants.
+ */
+#ifndef KG_ORACLE_H
+#define KG_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { KGO_METRIC_MINKOWSKI = 0, KGO_METRIC_KERR_SCHILD = 1, KGO_METRIC_MKS = 2, KGO_METRIC_SCHW_EF = 3 };
+enum { KGO_BC_OUTFLOW = 0, KGO_BC_INFLOW = 1 };
+enum { KGO_PIRK2 = 0, KGO_PIRK3 = 1, KGO_RK3 = 2, KGO_RK4 = 3 };
+
+typedef struct kgo_sim {
+  /* grid (mesh_refinement_lib.f90:100-147, nproc = 1) */
+  int Nx, Ny, Nz, g;
+  int ni, nj, nk;
+  int64_t npts;        /* ni*nj*nk */
+  double xmin, xmax, dx, dy, dz;
+  double *x, *y, *z;   /* x[ii], y[jj], z[kk] */
+  /* metric (geometry.f90:92-352), 2-D tables [jj*ni + ii] */
+  int metric;
+  double a, M_bh, m;
+  double *alpha, *beta_r, *dx_beta, *sqrtg;
+  double *g_rr, *g_rp, *g_tt, *g_pp;  /* gamma_up(1,1),(1,3),(2,2),(3,3) */
+  /* full 4x4 g_dn and g_up, 2-D tables, index [(mu*4+nu)*ni*nj + jj*ni + ii] */
+  double *g_dn, *g_up;
+  /* time stepping */
+  int bc_rmax;
+  double t, dt, CFL;
+  /* state: 5 fields each, npts doubles per field */
+  double *u, *u_p, *u_stage, *u_stage_2, *rhs_u, *k1, *k2, *k3, *k4;
+  /* scratch */
+  double *tmp[8];
+} kgo_sim;
+
+/* Creates a simulation.  dt <= 0 selects the reference formula
+ * (diagnostic_lib.f90:101-104).  Returns NULL on invalid input. */
+kgo_sim* kgo_create(int Nx, int Ny, int Nz, int g, double xmin, double xmax,
+                    int metric, double a, double M_bh, double m,
+                    int bc_rmax, double CFL, double dt);
+void kgo_destroy(kgo_sim* s);
+
+/* accessors for ctypes */
+double* kgo_field(kgo_sim* s, int which);   /* 0=u 1=u_p 2=u_stage 3=u_stage_2 4=rhs_u */
+double* kgo_table(kgo_sim* s, int which);   /* 0 alpha 1 beta_r 2 dx_beta 3 sqrtg 4 g_rr 5 g_rp 6 g_tt 7 g_pp 8 g_dn 9 g_up 10 x 11 y 12 z */
+void kgo_info(const kgo_sim* s, int64_t* out8, double* outd8);
+
+/* reference-formula dt (diagnostic_lib.f90:101-104) */
+double kgo_reference_dt(int Nx, int Ny, int Nz, int g, double xmin, double xmax, double CFL);
+
+/* Stencils on full padded arrays (finite_differences.f90). res = 0.5/h. */
+void kgo_dx1(const kgo_sim* s, const double* f, double* df);
+void kgo_dx2(const kgo_sim* s, const double* f, double* df);
+void kgo_dx3(const kgo_sim* s, const double* f, double* df);
+void kgo_advec_x(const kgo_sim* s, const double* f, const double* beta2d, double* df);
+/* Same stencils on caller-described arrays (for the reference cross-check). */
+void kgo_dx1_raw(int ni, int nj, int nk, double h, const double* f, double* df);
+void kgo_dx2_raw(int ni, int nj, int nk, double h, const double* f, double* df);
+void kgo_dx3_raw(int ni, int nj, int nk, double h, const double* f, double* df);
+void kgo_advec_x_raw(int ni, int nj, int nk, double h, const double* f, const double* beta3d, double* df);
+
+/* RHS pieces (lines_method_lib.f90:671-753). vars = 5-field state. */
+void kgo_rhs_phi(kgo_sim* s, double* rhs, const double* vars);
+void kgo_rhs_psi1(kgo_sim* s, double* rhs, const double* vars);
+void kgo_rhs_psi2(kgo_sim* s, double* rhs, const double* vars);
+void kgo_rhs_psi3(kgo_sim* s, double* rhs, const double* vars);
+void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars);
+void kgo_rhs_all(kgo_sim* s, double* rhs5, const double* vars);
+
+/* ghost fill of a 5-field state (lines_method_lib.f90:313-330,507-597) */
+void kgo_fill(kgo_sim* s, double* u, const double* u_p);
+
+/* integrators (lines_method_lib.f90:211-474); advance s->u by nsteps */
+void kgo_pirk2(kgo_sim* s, int nsteps);
+void kgo_pirk3(kgo_sim* s, int nsteps);
+void kgo_rk3(kgo_sim* s, int nsteps);
+void kgo_rk4(kgo_sim* s, int nsteps);
+void kgo_step(kgo_sim* s, int scheme, int nsteps);
+
+/* The integrators are written once over an operator table, so the stage
+ * arithmetic can be pinned on small linear systems as well as run with the
+ * Klein-Gordon operator.  n = values per field; a state is 5*n doubles.
+ *   rhs14: writes fields 1..4 (phi, psi_r, psi_theta, psi_phi) of R(vars)
+ *   rhs5:  writes field 5 (Pi) of R(vars) into out (n doubles)
+ *   fill:  ghost fill of u (u_p is the start-of-step state), may be NULL */
+typedef struct kgo_ops {
+  void (*rhs14)(void* ctx, const double* vars, double* out5);
+  void (*rhs5)(void* ctx, const double* vars, double* out1);
+  void (*fill)(void* ctx, double* u, const double* u_p);
+  void* ctx;
+  int64_t n;
+} kgo_ops;
+typedef struct kgo_bufs {
+  double *u, *u_p, *u_stage, *u_stage_2, *rhs_u, *k1, *k2, *k3, *k4;
+} kgo_bufs;
+void kgo_pirk2_generic(const kgo_ops* ops, kgo_bufs* b, double dt);
+void kgo_pirk3_generic(const kgo_ops* ops, kgo_bufs* b, double dt);
+void kgo_rk3_generic(const kgo_ops* ops, kgo_bufs* b, double dt);
+void kgo_rk4_generic(const kgo_ops* ops, kgo_bufs* b, double dt);
+
+/* Index-tag probe of the fill maps: writes, for every padded point of one
+ * field, the flat index of the evolved point its value is copied from after
+ * kgo_fill.  A source whose i index is -g stands for the xL extrapolation at
+ * that row; -1 stands for an inflow value. */
+void kgo_fill_source_map(kgo_sim* s, int64_t* src);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
