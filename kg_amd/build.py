@@ -32,6 +32,30 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FORTRAN, "_build")
+FEXE = os.path.join(FORTRAN, "kg_evolve")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The ISO_C_BINDING module and the example driver (needs amdflang)."""
+    fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+    if fc is None:
+        return None
+    os.makedirs(FBUILD, exist_ok=True)
+    cmds = [
+        [fc, "-O2", "-module-dir", FBUILD, "-c", os.path.join(FORTRAN, "kg_hip_mod.f90"), "-o",
+         os.path.join(FBUILD, "kg_hip_mod.o")],
+        [fc, "-O2", "-I", FBUILD, os.path.join(FORTRAN, "kg_evolve.f90"), os.path.join(FBUILD, "kg_hip_mod.o"),
+         "-L" + HERE, "-lkg_hip", "-Wl,-rpath,$ORIGIN/..", "-o", FEXE],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return FEXE
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + HIPFLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
@@ -39,6 +63,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 

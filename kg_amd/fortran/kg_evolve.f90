@@ -1,0 +1,58 @@
+! kg_evolve.f90 -- a Fortran driver in the shape of the reference's evolve
+! loop (main/evolve.f90:22-26), with the step done by the MI355X engine.
+!
+! usage: kg_evolve Nx Ny Nz xmin xmax m scheme nsteps in.bin out.bin
+!   scheme: pirk2 | pirk3 | rk3 | rk4
+!   in.bin/out.bin: u(5, -g:Nx+g, -g:Ny+g, 0:Nz) complex(8), raw stream
+program kg_evolve
+  use kg_hip_mod
+  implicit none
+  type(kg_lines_method) :: MoL
+  complex(kind=8), allocatable :: u(:,:,:,:)
+  character(len=256) :: arg, fin, fout, scheme
+  integer :: Nx, Ny, Nz, nsteps, l, g, isch
+  real(kind=8) :: xmin, xmax, m
+
+  g = 2
+  call get_command_argument(1, arg); read(arg, *) Nx
+  call get_command_argument(2, arg); read(arg, *) Ny
+  call get_command_argument(3, arg); read(arg, *) Nz
+  call get_command_argument(4, arg); read(arg, *) xmin
+  call get_command_argument(5, arg); read(arg, *) xmax
+  call get_command_argument(6, arg); read(arg, *) m
+  call get_command_argument(7, scheme)
+  call get_command_argument(8, arg); read(arg, *) nsteps
+  call get_command_argument(9, fin)
+  call get_command_argument(10, fout)
+
+  select case (trim(scheme))
+  case ('pirk2'); isch = KG_SCHEME_PIRK2
+  case ('pirk3'); isch = KG_SCHEME_PIRK3
+  case ('rk3');   isch = KG_SCHEME_RK3
+  case default;   isch = KG_SCHEME_RK4
+  end select
+
+  MoL = kg_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
+                                    'outflow', 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true.)
+  allocate(u(5, -g:Nx+g, -g:Ny+g, 0:Nz))
+  open(unit=10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(10) u
+  close(10)
+  call MoL%upload(u)
+
+  do l = 1, nsteps
+    select case (isch)
+    case (KG_SCHEME_PIRK2); call MoL%pirk2()
+    case (KG_SCHEME_PIRK3); call MoL%pirk3()
+    case (KG_SCHEME_RK3);   call MoL%rk3()
+    case default;    call MoL%rk4()
+    end select
+  end do
+
+  call MoL%download(u)
+  open(unit=11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(11) u
+  close(11)
+  print '(A,I8,A,ES24.16,A,ES24.16)', 'steps ', nsteps, ' t ', MoL%t, ' dt ', MoL%dt
+  call MoL%destroy()
+end program kg_evolve

@@ -1,0 +1,276 @@
+! kg_hip_mod.f90 -- ISO_C_BINDING layer between a Fortran driver of
+! FlavioRosales/KG and the MI355X engine (include/kg.h, kg_amd/libkg_hip.so).
+!
+! The reference driver holds `type(scalar_field) :: MoL` and calls
+! `MoL%pirk2()` once per step (main/evolve.f90:22-26); the state is
+! `MoL%u(nvars, iL-g:iR+g, jL-g:jR+g, kL:kR)`, complex(8)
+! (MoL/lines_method_lib.f90:27,135-138).  `type(kg_lines_method)` below keeps
+! that shape of use: construct it with the arguments of
+! lines_method_constructor (:49-51), hand it `u` with `upload`, call
+! `pirk2`/`pirk3`/`rk3`/`rk4` or `step(n)`, and read `u` back with
+! `download` when the diagnostics need it.  Every nonzero status prints the
+! engine's message and stops, the reference's error convention
+! (lines_method_lib.f90:529-533,744-748).
+module kg_hip_mod
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: KG_OK = 0
+  integer(c_int32_t), parameter, public :: KG_SCHEME_PIRK2 = 0, KG_SCHEME_PIRK3 = 1, KG_SCHEME_RK3 = 2, KG_SCHEME_RK4 = 3
+  integer(c_int32_t), parameter, public :: KG_METRIC_MINKOWSKI = 0, KG_METRIC_KERR_SCHILD = 1, &
+                                           KG_METRIC_MKS = 2, KG_METRIC_SCHW_EF = 3
+  integer(c_int32_t), parameter, public :: KG_BC_OUTFLOW = 0, KG_BC_INFLOW = 1
+  integer(c_int32_t), parameter, public :: KG_LAYOUT_FORTRAN = 0, KG_LAYOUT_SOA = 1
+
+  type, bind(C), public :: kg_desc
+    integer(c_int32_t) :: nvars, g_pts, Nx, Ny, Nz
+    integer(c_int32_t) :: scheme, metric, bc_rmax, is_complex, reserved
+    real(c_double) :: xmin, xmax, tmin, CFL, dt, a, M_bh, m
+  end type kg_desc
+
+  interface
+    function kg_create(desc, h) bind(C, name="kg_create") result(st)
+      import :: kg_desc, c_ptr, c_int32_t
+      type(kg_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: h
+      integer(c_int32_t) :: st
+    end function kg_create
+    function kg_destroy(h) bind(C, name="kg_destroy") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: st
+    end function kg_destroy
+    function kg_last_error(h) bind(C, name="kg_last_error") result(msg)
+      import :: c_ptr
+      type(c_ptr), value :: h
+      type(c_ptr) :: msg
+    end function kg_last_error
+    function kg_get_time(h, t, dt) bind(C, name="kg_get_time") result(st)
+      import :: c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: t, dt
+      integer(c_int32_t) :: st
+    end function kg_get_time
+    function kg_set_time(h, t) bind(C, name="kg_set_time") result(st)
+      import :: c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), value :: t
+      integer(c_int32_t) :: st
+    end function kg_set_time
+    function kg_upload_state(h, u, layout) bind(C, name="kg_upload_state") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h, u
+      integer(c_int32_t), value :: layout
+      integer(c_int32_t) :: st
+    end function kg_upload_state
+    function kg_download_state(h, u, layout) bind(C, name="kg_download_state") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h, u
+      integer(c_int32_t), value :: layout
+      integer(c_int32_t) :: st
+    end function kg_download_state
+    function kg_upload_aux(h, which, u, layout) bind(C, name="kg_upload_aux") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h, u
+      integer(c_int32_t), value :: which, layout
+      integer(c_int32_t) :: st
+    end function kg_upload_aux
+    function kg_download_aux(h, which, u, layout) bind(C, name="kg_download_aux") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h, u
+      integer(c_int32_t), value :: which, layout
+      integer(c_int32_t) :: st
+    end function kg_download_aux
+    function kg_fill_ghosts(h) bind(C, name="kg_fill_ghosts") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: st
+    end function kg_fill_ghosts
+    function kg_pirk2(h) bind(C, name="kg_pirk2") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: st
+    end function kg_pirk2
+    function kg_pirk3(h) bind(C, name="kg_pirk3") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: st
+    end function kg_pirk3
+    function kg_rk3(h) bind(C, name="kg_rk3") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: st
+    end function kg_rk3
+    function kg_rk4(h) bind(C, name="kg_rk4") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: st
+    end function kg_rk4
+    function kg_step(h, nsteps) bind(C, name="kg_step") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsteps
+      integer(c_int32_t) :: st
+    end function kg_step
+    function kg_sync(h) bind(C, name="kg_sync") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: st
+    end function kg_sync
+  end interface
+
+  public :: kg_create, kg_destroy, kg_last_error, kg_get_time, kg_set_time, kg_upload_state, &
+            kg_download_state, kg_upload_aux, kg_download_aux, kg_fill_ghosts, kg_pirk2, kg_pirk3, &
+            kg_rk3, kg_rk4, kg_step, kg_sync
+
+  ! The reference's lines_method as seen by its driver.
+  type, public :: kg_lines_method
+    type(c_ptr) :: h = c_null_ptr
+    integer :: nvars, g_pts, Nx, Ny, Nz
+    integer :: iL, iR, jL, jR, kL, kR
+    real(kind=8) :: tmin, tmax, CFL, dt, t
+    integer :: Nt
+  contains
+    procedure :: upload => klm_upload
+    procedure :: download => klm_download
+    procedure :: boundary => klm_boundary
+    procedure :: pirk2 => klm_pirk2
+    procedure :: pirk3 => klm_pirk3
+    procedure :: rk3 => klm_rk3
+    procedure :: rk4 => klm_rk4
+    procedure :: step => klm_step
+    procedure :: destroy => klm_destroy
+  end type kg_lines_method
+
+  public :: kg_lines_method_constructor
+
+contains
+
+  subroutine kg_check(st, h)
+    integer(c_int32_t), intent(in) :: st
+    type(c_ptr), intent(in) :: h
+    character(kind=c_char), pointer :: msg(:)
+    type(c_ptr) :: p
+    integer :: n
+    if (st == KG_OK) return
+    p = kg_last_error(h)
+    call c_f_pointer(p, msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    print*, 'kg engine error ', st, ': ', transfer(msg(1:n), repeat(' ', n))
+    stop
+  end subroutine kg_check
+
+  ! lines_method_constructor(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, tmax,
+  ! CFL, boundary_type_rmax, metric, a, M_bh, m) -- lines_method_lib.f90:49-129
+  function kg_lines_method_constructor(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, tmax, CFL, &
+                                       boundary_type_rmax, metric, a, M_bh, m, scheme, is_complex) result(this)
+    integer, intent(in) :: nvars, g_pts, Nx, Ny, Nz
+    real(kind=8), intent(in) :: xmin, xmax, tmin, tmax, CFL, a, M_bh, m
+    character(len=*), intent(in) :: boundary_type_rmax, metric
+    integer, intent(in) :: scheme
+    logical, intent(in) :: is_complex
+    type(kg_lines_method) :: this
+    type(kg_desc) :: d
+    real(c_double) :: t, dt
+    d%nvars = nvars; d%g_pts = g_pts; d%Nx = Nx; d%Ny = Ny; d%Nz = Nz
+    d%scheme = scheme
+    if (metric == 'Kerr_Schild') then
+      d%metric = KG_METRIC_KERR_SCHILD
+    else if (metric == 'Schw_EF') then
+      d%metric = KG_METRIC_SCHW_EF
+    else if (metric == 'MKS') then
+      d%metric = KG_METRIC_MKS
+    else
+      d%metric = KG_METRIC_MINKOWSKI
+    end if
+    if (boundary_type_rmax == 'outflow') then
+      d%bc_rmax = KG_BC_OUTFLOW
+    else if (boundary_type_rmax == 'inflow') then
+      d%bc_rmax = KG_BC_INFLOW
+    else
+      print*, 'This boundary condition type for r_max is not implemented yet'
+      print*, 'try with: outflow or inflow'
+      stop
+    end if
+    d%is_complex = merge(1, 0, is_complex)
+    d%reserved = 0
+    d%xmin = xmin; d%xmax = xmax; d%tmin = tmin; d%CFL = CFL; d%dt = 0.0d0
+    d%a = a; d%M_bh = M_bh; d%m = m
+    call kg_check(kg_create(d, this%h), c_null_ptr)
+    this%nvars = nvars; this%g_pts = g_pts; this%Nx = Nx; this%Ny = Ny; this%Nz = Nz
+    this%iL = 0; this%iR = Nx; this%jL = 0; this%jR = Ny; this%kL = 0; this%kR = Nz
+    call kg_check(kg_get_time(this%h, t, dt), this%h)
+    this%t = t; this%dt = dt
+    this%tmin = tmin; this%tmax = tmax; this%CFL = CFL
+    this%Nt = int((tmax - tmin) / dt)           ! lines_method_lib.f90:115-116
+    if (mod(this%Nt, 2) /= 0) this%Nt = this%Nt + 1
+  end function kg_lines_method_constructor
+
+  subroutine klm_upload(this, u)
+    class(kg_lines_method), intent(inout) :: this
+    complex(kind=8), intent(in), target :: u(:,:,:,:)
+    call kg_check(kg_upload_state(this%h, c_loc(u), KG_LAYOUT_FORTRAN), this%h)
+  end subroutine klm_upload
+
+  subroutine klm_download(this, u)
+    class(kg_lines_method), intent(inout) :: this
+    complex(kind=8), intent(inout), target :: u(:,:,:,:)
+    call kg_check(kg_download_state(this%h, c_loc(u), KG_LAYOUT_FORTRAN), this%h)
+  end subroutine klm_download
+
+  subroutine klm_boundary(this)
+    class(kg_lines_method), intent(inout) :: this
+    call kg_check(kg_fill_ghosts(this%h), this%h)
+  end subroutine klm_boundary
+
+  subroutine sync_time(this)
+    class(kg_lines_method), intent(inout) :: this
+    real(c_double) :: t, dt
+    call kg_check(kg_get_time(this%h, t, dt), this%h)
+    this%t = t
+  end subroutine sync_time
+
+  subroutine klm_pirk2(this)
+    class(kg_lines_method), intent(inout) :: this
+    call kg_check(kg_pirk2(this%h), this%h)
+    call sync_time(this)
+  end subroutine klm_pirk2
+
+  subroutine klm_pirk3(this)
+    class(kg_lines_method), intent(inout) :: this
+    call kg_check(kg_pirk3(this%h), this%h)
+    call sync_time(this)
+  end subroutine klm_pirk3
+
+  subroutine klm_rk3(this)
+    class(kg_lines_method), intent(inout) :: this
+    call kg_check(kg_rk3(this%h), this%h)
+    call sync_time(this)
+  end subroutine klm_rk3
+
+  subroutine klm_rk4(this)
+    class(kg_lines_method), intent(inout) :: this
+    call kg_check(kg_rk4(this%h), this%h)
+    call sync_time(this)
+  end subroutine klm_rk4
+
+  subroutine klm_step(this, n)
+    class(kg_lines_method), intent(inout) :: this
+    integer, intent(in) :: n
+    call kg_check(kg_step(this%h, int(n, c_int32_t)), this%h)
+    call sync_time(this)
+  end subroutine klm_step
+
+  subroutine klm_destroy(this)
+    class(kg_lines_method), intent(inout) :: this
+    integer(c_int32_t) :: st
+    if (c_associated(this%h)) st = kg_destroy(this%h)
+    this%h = c_null_ptr
+  end subroutine klm_destroy
+
+end module kg_hip_mod

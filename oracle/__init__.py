@@ -195,3 +195,60 @@ def ref_stencil(name, f, h, g, beta=None):
     else:
         getattr(_ref, "kgref_" + name)(ni, nj, nk, g, h, f.ctypes.data, out.ctypes.data)
     return out
+
+
+# --------------------------------------------------------------------------
+# The integrators' stage arithmetic with a caller-supplied right-hand side
+# (kgo_*_generic), for pinning the combine formulas on linear systems.
+# --------------------------------------------------------------------------
+_RHS_CB = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp)
+_FILL_CB = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp)
+
+
+class _Ops(C.Structure):
+    _fields_ = [("rhs14", _RHS_CB), ("rhs5", _RHS_CB), ("fill", C.c_void_p), ("ctx", C.c_void_p),
+                ("n", C.c_int64)]
+
+
+class _Bufs(C.Structure):
+    _fields_ = [(nm, _dp) for nm in ("u", "u_p", "u_stage", "u_stage_2", "rhs_u", "k1", "k2", "k3", "k4")]
+
+
+class GenericIntegrator:
+    """Runs one of the oracle's integrators on n values per field with Python
+    right-hand sides: rhs14(vars[5,n]) -> out[4,n], rhs5(vars[5,n]) -> out[n]."""
+
+    def __init__(self, n, rhs14, rhs5):
+        self.n = n
+        names = [f[0] for f in _Bufs._fields_]
+        self.bufs = {nm: np.zeros(5 * n) for nm in names}
+
+        def c14(ctx, vars_p, out_p):
+            v = np.ctypeslib.as_array(vars_p, shape=(5, n))
+            o = np.ctypeslib.as_array(out_p, shape=(5, n))
+            o[:4] = rhs14(v.copy())
+
+        def c5(ctx, vars_p, out_p):
+            v = np.ctypeslib.as_array(vars_p, shape=(5, n))
+            o = np.ctypeslib.as_array(out_p, shape=(n,))
+            o[:] = rhs5(v.copy())
+
+        self._c14 = _RHS_CB(c14)
+        self._c5 = _RHS_CB(c5)
+        self.ops = _Ops(self._c14, self._c5, None, None, n)
+        self.b = _Bufs(*[self.bufs[nm].ctypes.data_as(_dp) for nm in names])
+        L = lib()
+        for nm in ("kgo_pirk2_generic", "kgo_pirk3_generic", "kgo_rk3_generic", "kgo_rk4_generic"):
+            getattr(L, nm).argtypes = [C.POINTER(_Ops), C.POINTER(_Bufs), C.c_double]
+
+    @property
+    def u(self):
+        return self.bufs["u"].reshape(5, self.n)
+
+    def stage_array(self, which=0):
+        return self.bufs["u_stage" if which == 0 else "u_stage_2"].reshape(5, self.n)
+
+    def step(self, scheme, dt, nsteps=1):
+        fn = getattr(lib(), "kgo_%s_generic" % scheme)
+        for _ in range(nsteps):
+            fn(C.byref(self.ops), C.byref(self.b), dt)
