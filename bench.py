@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kc", type=int, default=0, help="fix the stage kernel's phi planes per workgroup (skips autotune)")
+    ap.add_argument("--tj", type=int, default=8, help="theta rows per workgroup when --kc is given")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (a single grid lives on one MI355X); "
@@ -81,6 +83,15 @@ def main():
                             CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"])
     lm.set_u_soa(u0[None])
     del u0
+    lm.boundary()  # ghost-fill the initial data (the reference's IC fills x, y, z ghosts the same way)
+    if args.kc > 0:
+        best = (args.kc, args.tj)
+        lm.set_tuning(*best)
+    else:
+        from kg_amd.autotune import autotune
+        best, timings = autotune(lm, steps=2)
+        print("bench: autotune %s" % {"%d/%d" % k: round(v, 4) for k, v in timings.items()}, file=sys.stderr,
+              flush=True)
     print("bench: warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
     lm.step(args.warmup)
     lm.sync()
@@ -124,7 +135,8 @@ def main():
         "data": "synthetic (seeded Gaussian shell x sum of Y_lm, l<=4)",
         "config": {"workload": "plain single-grid RK4, Nr=2048 r in [1.8,200], Ntheta=128, Nphi=256, m=0.1, "
                                "Kerr-Schild M=1, CFL 0.25",
-                   "grid_points": npts, "scheme": "rk4", "fields": 5, "state_finite": finite},
+                   "grid_points": npts, "scheme": "rk4", "fields": 5, "state_finite": finite,
+                   "tuning": {"kc": best[0], "tj": best[1]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "stage_kernel (RHS + RK4 stage combine)",

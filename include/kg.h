@@ -128,6 +128,11 @@ kg_status kg_step(kg_handle* h, int32_t nsteps);
 /* waits for the handle's stream */
 kg_status kg_sync(kg_handle* h);
 
+/* Launch shape of the stage kernel (no reference counterpart): kc = phi
+ * planes each workgroup marches over, tj = theta rows per workgroup (4 or 8).
+ * Results do not depend on it; kg_autotune-style callers sweep it. */
+kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj);
+
 /* Kernel timing with HIP events on the handle's stream (no reference
  * counterpart; the reference only prints MPI_Wtime, save_gnuplot.f90:154).
  * kind: 0 = RHS+stage kernels, 1 = ghost-fill kernels.  While enabled,

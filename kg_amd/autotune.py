@@ -1,0 +1,44 @@
+"""Launch-shape autotuner for the stage kernel.
+
+The stage kernel's workgroup covers 64 r-points x `tj` theta rows and marches
+over `kc` phi planes.  The best shape depends on the grid (how many
+workgroups fill the 256 CUs, how much halo each tile re-reads), so it is
+measured: every candidate runs a few steps on the caller's own handle with
+HIP events around each stage kernel, and the fastest is kept.  Results do
+not depend on the shape (the kernel computes the same expressions in the
+same order for any tiling), so tuning can run on live state: the state is
+saved before and restored after.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+CANDIDATES = [(kc, tj) for tj in (4, 8) for kc in (8, 16, 32, 64)]
+
+
+def autotune(lm, candidates=CANDIDATES, steps=2, verbose=False):
+    """Pick (kc, tj) for LinesMethod `lm`; returns (best, {shape: ms per stage kernel})."""
+    saved = lm.u_soa()
+    t_saved = lm.t
+    aux = None
+    if lm.desc.scheme in (0, 1):  # PIRK: the stale stage arrays carry over
+        aux = [lm.u_stage_soa(0)] + ([lm.u_stage_soa(1)] if lm.desc.scheme == 1 else [])
+    timings = {}
+    for kc, tj in candidates:
+        lm.set_tuning(kc, tj)
+        lm.step(1)  # warm-up (graph capture)
+        lm.profile(True)
+        lm.step(steps)
+        ms, n = lm.profile_read(0)
+        lm.profile(False)
+        timings[(kc, tj)] = ms / max(n, 1)
+        if verbose:
+            print("autotune kc=%d tj=%d: %.4f ms/stage" % (kc, tj, timings[(kc, tj)]), flush=True)
+    best = min(timings, key=timings.get)
+    lm.set_tuning(*best)
+    lm.set_u_soa(saved)
+    lm.t = t_saved
+    if aux is not None:
+        for w, arr in enumerate(aux):
+            lm.set_u_stage_soa(w, arr)
+    return best, timings

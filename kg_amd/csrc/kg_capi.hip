@@ -49,6 +49,8 @@ struct kg_handle {
   double* S25[2] = {nullptr, nullptr};
   State5 Ua, Ub, A;     // RK stage states and the RK4 accumulator
   int p5 = 0, p25 = 0;  // slot holding the current u_stage(5), u_stage_2(5)
+  int tune_kc = 32;     // phi planes per workgroup of the stage kernel
+  int tune_tj = 4;      // theta rows per workgroup
   double t = 0.0;
   std::string err;
   std::vector<void*> allocs;
@@ -168,7 +170,8 @@ kg_status stage(kg_handle* h, int mode, const Box& b, const State5& in, const St
   a.j_hi = b.j_hi;
   a.k_lo = b.k_lo;
   a.k_hi = b.k_hi;
-  a.kc = 8;
+  a.kc = h->tune_kc;
+  a.tj = h->tune_tj;
   a.nsys = h->nsys;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof) {
@@ -433,6 +436,11 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
   G.plane = (int64_t)G.ld * G.nj;
   G.fstride = (G.plane * G.nk + 31) / 32 * 32;
   G.sys_stride = G.fstride;
+  if (G.fstride * 8 * h->nsys >= (int64_t(1) << 31)) {
+    fail(nullptr, KG_ERR_INVALID, "grid too large: one field must stay below 2 GiB");
+    delete h;
+    return KG_ERR_INVALID;
+  }
   G.rr = 0.50 / h->hg.dx;
   G.rt = 0.50 / h->hg.dy;
   G.rp = 0.50 / h->hg.dz;
@@ -573,6 +581,17 @@ kg_status kg_step(kg_handle* h, int32_t nsteps) {
 kg_status kg_sync(kg_handle* h) {
   if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
   KG_HIP(h, hipStreamSynchronize(h->stream));
+  return KG_OK;
+}
+
+kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  if (kc < 1 || kc > 4096 || (tj != 4 && tj != 8)) return fail(h, KG_ERR_INVALID, "tuning: kc in [1,4096], tj in {4,8}");
+  KG_HIP(h, hipStreamSynchronize(h->stream));
+  h->tune_kc = kc;
+  h->tune_tj = tj;
+  for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
   return KG_OK;
 }
 

@@ -80,7 +80,8 @@ struct StageArgs {
   double* out[5];           // stage output
   double* out2[5];          // RK4 accumulator output
   int i_lo, i_hi, j_lo, j_hi, k_lo, k_hi;  // computed box, [lo, hi)
-  int kc;                   // k-chunk per workgroup
+  int kc;                   // phi planes per workgroup
+  int tj;                   // theta rows per workgroup (4 or 8)
   int nti, ntj, ntk;        // tile counts (set by launch_stage)
   int nsys;                 // independent real systems sharing the grid
 };

@@ -20,194 +20,371 @@
 namespace kg {
 namespace {
 
-constexpr int TI = 64;  // tile width along r (one wavefront)
-constexpr int TJ = 4;   // tile height along theta
-
-struct View {
-  const double* __restrict__ tab;
-  int64_t n2;  // nj*ld
-  int ld;
-  int64_t plane;
-  __device__ __forceinline__ double t(int which, int ii, int jj) const {
-    return tab[which * n2 + (int64_t)jj * ld + ii];
-  }
-  __device__ __forceinline__ double f(const double* __restrict__ p, int ii, int jj, int kk) const {
-    return p[(int64_t)kk * plane + (int64_t)jj * ld + ii];
-  }
-};
-
-// D_r (first_derivative_x_2_uniform, finite_differences.f90:36-42)
-template <class Q>
-__device__ __forceinline__ double d_r(const Q& q, int ii, int ni, double res) {
-  if (ii == 0) return (-3.0 * q(0) + 4.0 * q(1) - q(2)) * res;
-  if (ii == ni - 1) return (+3.0 * q(ni - 1) - 4.0 * q(ni - 2) + q(ni - 3)) * res;
-  return (-q(ii - 1) + q(ii + 1)) * res;
+// Global access as (uniform 64-bit base) + (32-bit byte offset): the base
+// stays in SGPRs and a lane carries one 32-bit offset, which maps to the
+// SGPR-base form of global_load/global_store.  Every field of one system is
+// below 2 GiB (checked at kg_create), so offsets fit.
+__device__ __forceinline__ double gld(const double* base, uint32_t off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void gst(double* base, uint32_t off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off) = v;
 }
 
-// A_r (advec_x_uniform, finite_differences.f90:779-799)
-template <class Q>
-__device__ __forceinline__ double a_r(const Q& q, int ii, int ni, double beta, double res) {
-  if (ii <= 1) return (-3.0 * q(ii) + 4.0 * q(ii + 1) - q(ii + 2)) * res;
-  if (ii == ni - 2) return (q(ni - 1) - q(ni - 3)) * res;
-  if (ii == ni - 1) return (3.0 * q(ni - 1) - 4.0 * q(ni - 2) + q(ni - 3)) * res;
-  if (beta > 0.0) return (-3.0 * q(ii) + 4.0 * q(ii + 1) - q(ii + 2)) * res;
-  if (beta < 0.0) return (3.0 * q(ii) - 4.0 * q(ii - 1) + q(ii - 2)) * res;
-  return (q(ii + 1) - q(ii - 1)) * res;
-}
-
-// D_theta (first_derivative_y_2 with inpar = .false., finite_differences.f90:214-230)
-template <class Q>
-__device__ __forceinline__ double d_th(const Q& q, int jj, int kk, int nj, int kt, double res) {
-  if (jj == 0) return (-q(1, kk) + q(2, kt)) * res;
-  if (jj == nj - 1) return (+3.0 * q(nj - 1, kk) - 4.0 * q(nj - 2, kk) + q(nj - 3, kk)) * res;
-  return (-q(jj - 1, kk) + q(jj + 1, kk)) * res;
-}
-
-// D_phi (first_derivative_z_2, finite_differences.f90:328-335)
-template <class Q>
-__device__ __forceinline__ double d_ph(const Q& q, int kk, int Nz, double res) {
-  if (kk == 0) return (-q(Nz - 2) + q(1)) * res;
-  if (kk == Nz) return (-q(2) + q(Nz - 1)) * res;
-  return (-q(kk - 1) + q(kk + 1)) * res;
-}
+constexpr int TI = 64;  // tile width along r: one wavefront per theta row
+constexpr int HALO_R = 2;  // r-neighbours needed on each side (A_r reaches i-2 and i+2)
+constexpr int NXCD = 8;    // XCDs of an MI355X; workgroups are dealt round-robin over them
 
 template <int MODE>
 struct ModeTraits {
-  static constexpr bool need14 = !(MODE == M_PIRK_1B || MODE == M_PIRK3_2B);
-  static constexpr bool need5 = !(MODE == M_PIRK_1A || MODE == M_PIRK3_2A);
+  // fields 1..4 (phi, psi_r, psi_theta, psi_phi) and/or field 5 (Pi) of the RHS
+  static constexpr bool n14 = !(MODE == M_PIRK_1B || MODE == M_PIRK3_2B);
+  static constexpr bool n5 = !(MODE == M_PIRK_1A || MODE == M_PIRK3_2A);
 };
 
-template <int MODE, int EF>
+__device__ __forceinline__ int remap_xcd(int b, int nb) {
+  // Workgroup b runs on XCD b % NXCD.  Give each XCD a contiguous run of
+  // tiles so tiles adjacent in theta (which share halo rows) meet in the
+  // same L2.  Bijective on [0, nb).
+  const int q = nb / NXCD, r = nb % NXCD;
+  const int x = b % NXCD, idx = b / NXCD;
+  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+// Stage kernel.  A workgroup owns a TI x TJ tile of (r, theta) columns and
+// marches over a chunk of phi planes.  At each plane every thread loads its
+// point's five fields, forms the products the stencils act on (alpha*Pi,
+// psi_r, the r-flux, Pi*sqrtg, F = alpha*Pi + beta*psi_r, the theta-flux)
+// with its own metric coefficients and shares them through LDS; r- and
+// theta-neighbours are read back from LDS, phi-neighbours come from a
+// register queue (the point's own F and phi-flux at k-1, k, k+1).  Halo
+// columns (i0-2, i0-1, i0+TI, i0+TI+1) and rows (j0-1, j0+TJ) are formed by
+// designated threads of the same workgroup.
+template <int MODE>
+struct Combine {
+  // operands of the stage combine read at the point: base (u_p, P, u) and
+  // aux (RK4 accumulator, PIRK stage state); fields [lo, hi)
+  static constexpr int lo = (MODE == M_PIRK_1B || MODE == M_PIRK3_2B) ? 4 : 0;
+  static constexpr int hi = (MODE == M_PIRK_1A || MODE == M_PIRK3_2A) ? 4 : 5;
+  static constexpr bool aux = MODE == M_RK4_S2 || MODE == M_RK4_S3 || MODE == M_RK4_S4 || MODE == M_PIRK3_2A ||
+                              MODE == M_PIRK3_2B;
+  static constexpr bool base = MODE != M_RK4_S1 || true;
+};
+
+template <int MODE, int EF, int TJ>
 __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
+  constexpr bool N14 = ModeTraits<MODE>::n14;
+  constexpr bool N5 = ModeTraits<MODE>::n5;
+  constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
+  constexpr bool CAUX = Combine<MODE>::aux;
+  constexpr int W = TI + 2 * HALO_R;
+  __shared__ double sG[TJ][W];    // alpha*Pi
+  __shared__ double sPR[TJ][W];   // psi_r
+  __shared__ double sFR[TJ][W];   // r-flux of rhs_pi
+  __shared__ double sPS[TJ][W];   // Pi*sqrtg
+  __shared__ double sF[TJ + 2][TI];   // alpha*Pi + beta*psi_r
+  __shared__ double sFT[TJ + 2][TI];  // theta-flux of rhs_pi
+
   const Geom& G = a.G;
-  const int b = blockIdx.x;
-  const int ti = b % a.nti;
-  const int rem = b / a.nti;
-  const int tj = rem % a.ntj;
-  const int tk = rem / a.ntj;
-  const int ii = a.i_lo + ti * TI + (int)threadIdx.x;
-  const int jj = a.j_lo + tj * TJ + (int)threadIdx.y;
-  if (ii >= a.i_hi || jj >= a.j_hi) return;
+  const int nb = a.nti * a.ntj * a.ntk;
+  const int tile = remap_xcd((int)blockIdx.x, nb);
+  const int ti = tile % a.nti;
+  const int tj = (tile / a.nti) % a.ntj;
+  const int tk = tile / (a.nti * a.ntj);
+  const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
+  const int tid = row * TI + lane;
+  const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
+  const int ii = i0 + lane, jj = j0 + row;
+  const int ni = G.ni, nj = G.nj, Nz = G.Nz, ld = G.ld;
+  const uint32_t plane8 = (uint32_t)(G.plane * 8);
+  const uint32_t n2b = (uint32_t)((int64_t)nj * ld * 8);  // bytes of one table
+  const bool valid = ii < ni && jj < nj;
+  const bool act = valid && ii < a.i_hi && jj < a.j_hi;
   const int k0 = a.k_lo + tk * a.kc;
   const int k1 = min(k0 + a.kc, a.k_hi);
-  const int64_t sys = (int64_t)blockIdx.y * G.sys_stride;  // system (complex: real, imaginary)
+  const int64_t sys = (int64_t)blockIdx.y * G.sys_stride;
+  const double* f_phi = a.in[0] + sys;
+  const double* f_psr = a.in[1] + sys;
+  const double* f_pst = a.in[2] + sys;
+  const double* f_psp = a.in[3] + sys;
+  const double* f_pi = a.in[4] + sys;
+  const double* tab = a.tab;
+  auto T = [&](int which, uint32_t off) { return gld(tab, off + (uint32_t)which * n2b); };
 
-  const View V{a.tab, (int64_t)G.nj * G.ld, G.ld, G.plane};
-  const int ni = G.ni, nj = G.nj, Nz = G.Nz;
-  const double dt = G.dt;
+  // own coefficients (the theta-ghost row of the tables is already the copy)
+  const uint32_t own = valid ? (uint32_t)((jj * ld + ii) * 8) : 0u;
+  const double al = T(T_ALPHA, own), be = T(T_BETA, own);
+  const double dxb = T(T_DXBETA, own), sg = T(T_SQRTG, own);
+  const double asg = T(T_ASG, own), grr = T(T_GRR, own);
+  const double grp = T(T_GRP, own), asgtt = T(T_ASGTT, own);
+  const double gpp = T(T_GPP, own), am2 = T(T_AM2, own);
 
-  const double* __restrict__ f_phi = a.in[0] + sys;
-  const double* __restrict__ f_psr = a.in[1] + sys;
-  const double* __restrict__ f_pst = a.in[2] + sys;
-  const double* __restrict__ f_psp = a.in[3] + sys;
-  const double* __restrict__ f_pi = a.in[4] + sys;
+  auto Fof = [&](double Pi, double psr) { return al * Pi + be * psr; };
+  auto fpof = [&](double psr, double psp) {
+    if constexpr (EF)
+      return (al * gpp) * psp;
+    else
+      return asg * (grp * psr + gpp * psp);
+  };
 
-  // coefficients of this (r, theta) point
-  const double al = V.t(T_ALPHA, ii, jj);
-  const double be = V.t(T_BETA, ii, jj);
-  const double dxb = V.t(T_DXBETA, ii, jj);
-  const double sg = V.t(T_SQRTG, ii, jj);
-  const double asg = V.t(T_ASG, ii, jj);
-  const double grp = V.t(T_GRP, ii, jj);
-  const double gpp = V.t(T_GPP, ii, jj);
-  const double am2 = V.t(T_AM2, ii, jj);
+  // halo task of this thread: 1 = theta row, 2 = r column, 0 = none
+  int hkind = 0, hi = 0, hj = 0, hrow = 0, hcol = 0;
+  if (tid < 2 * TI) {
+    hkind = 1;
+    hi = i0 + (tid % TI);
+    hj = tid < TI ? j0 - 1 : j0 + TJ;
+    hrow = tid < TI ? 0 : TJ + 1;
+    hcol = tid % TI;
+  } else if (tid < 2 * TI + 4 * TJ) {
+    const int u = tid - 2 * TI;
+    const int cc = u % 4;
+    hkind = 2;
+    hj = j0 + u / 4;
+    hrow = u / 4;
+    hi = cc == 0 ? i0 - 2 : cc == 1 ? i0 - 1 : cc == 2 ? i0 + TI : i0 + TI + 1;
+    hcol = hi - i0 + HALO_R;
+  }
+  if (hi < 0 || hi >= ni || hj < 0 || hj >= nj) hkind = 0;
+  const uint32_t hown = hkind ? (uint32_t)((hj * ld + hi) * 8) : 0u;
+  // halo coefficients: theta rows use alpha, beta, (alpha sqrtg) gamma^tt;
+  // r columns use alpha, alpha*sqrtg, gamma^rr, gamma^rphi, sqrtg
+  double h0 = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
+  if (hkind == 1) {
+    h0 = T(T_ALPHA, hown);
+    h1 = T(T_BETA, hown);
+    h2 = T(T_ASGTT, hown);
+  } else if (hkind == 2) {
+    h0 = T(T_ALPHA, hown);
+    h1 = T(T_ASG, hown);
+    h2 = T(T_GRR, hown);
+    h3 = T(T_GRP, hown);
+    h4 = T(T_SQRTG, hown);
+  }
+  // halo raw values: Pi, psi_r and psi_theta (theta rows) or psi_phi (r columns)
+  auto load_halo = [&](uint32_t sk, double& x0, double& x1, double& x2) {
+    if (hkind == 1) {
+      if constexpr (N14) {
+        x0 = gld(f_pi, hown + sk);
+        x1 = gld(f_psr, hown + sk);
+      }
+      if constexpr (N5) x2 = gld(f_pst, hown + sk);
+    } else if (hkind == 2) {
+      x0 = gld(f_pi, hown + sk);
+      x1 = gld(f_psr, hown + sk);
+      if constexpr (N5 && !EF) x2 = gld(f_psp, hown + sk);
+    }
+  };
+
+  // phi queue at the own point: F and the phi-flux at k-1 and k
+  double Pi = 0, psr = 0, pst = 0, psp = 0;  // own values at plane kk
+  double Fm = 0, fpm = 0, F0 = 0, fp0 = 0;
+  double hx0 = 0, hx1 = 0, hx2 = 0;  // halo values at plane kk
+  if (valid) {
+    const int km = k0 == 0 ? Nz - 2 : k0 - 1;  // D_phi at k = 0 pairs f(Nz-2) with f(1)
+    const uint32_t sm = (uint32_t)km * plane8 + own;
+    Fm = Fof(gld(f_pi, sm), gld(f_psr, sm));
+    if constexpr (N5) fpm = fpof(gld(f_psr, sm), gld(f_psp, sm));
+    const uint32_t s0 = (uint32_t)k0 * plane8 + own;
+    Pi = gld(f_pi, s0);
+    psr = gld(f_psr, s0);
+    if constexpr (N5) {
+      pst = gld(f_pst, s0);
+      psp = gld(f_psp, s0);
+      fp0 = fpof(psr, psp);
+    }
+    F0 = Fof(Pi, psr);
+  }
+  load_halo((uint32_t)k0 * plane8, hx0, hx1, hx2);
 
   const int dis = Nz / 2;
-  for (int kk = k0; kk < k1; ++kk) {
-    const int kt = (kk + dis) % (Nz - 1);
-    double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const double Pi = V.f(f_pi, ii, jj, kk);
-    const double psr = V.f(f_psr, ii, jj, kk);
-    if constexpr (ModeTraits<MODE>::need14) {
-      // rhs_phi (:675)
-      R[0] = al * Pi + be * psr;
-      // rhs_psi1 (:687-689)
-      auto aPi = [&](int i) { return V.t(T_ALPHA, i, jj) * V.f(f_pi, i, jj, kk); };
-      auto psr_r = [&](int i) { return V.f(f_psr, i, jj, kk); };
-      R[1] = d_r(aPi, ii, ni, G.rr) + be * a_r(psr_r, ii, ni, be, G.rr) + dxb * psr;
-      // rhs_psi2 (:700) and rhs_psi3 (:711): D(alpha*Pi + beta*psi_r)
-      auto Fth = [&](int j, int k) {
-        return V.t(T_ALPHA, ii, j) * V.f(f_pi, ii, j, k) + V.t(T_BETA, ii, j) * V.f(f_psr, ii, j, k);
-      };
-      auto Fph = [&](int k) { return al * V.f(f_pi, ii, jj, k) + be * V.f(f_psr, ii, jj, k); };
-      R[2] = d_th(Fth, jj, kk, nj, kt, G.rt);
-      R[3] = d_ph(Fph, kk, Nz, G.rp);
-    }
-    if constexpr (ModeTraits<MODE>::need5) {
-      // rhs_pi (:728-736 Kerr_Schild; :738-743 Schw_EF)
-      auto fr = [&](int i) {
-        if constexpr (EF)
-          return V.t(T_ASG, i, jj) * V.t(T_GRR, i, jj) * V.f(f_psr, i, jj, kk);
-        else
-          return V.t(T_ASG, i, jj) *
-                 (V.t(T_GRR, i, jj) * V.f(f_psr, i, jj, kk) + V.t(T_GRP, i, jj) * V.f(f_psp, i, jj, kk));
-      };
-      auto ft = [&](int j, int k) { return V.t(T_ASGTT, ii, j) * V.f(f_pst, ii, j, k); };
-      auto fp = [&](int k) {
-        if constexpr (EF)
-          return (al * gpp) * V.f(f_psp, ii, jj, k);
-        else
-          return asg * (grp * V.f(f_psr, ii, jj, k) + gpp * V.f(f_psp, ii, jj, k));
-      };
-      auto ps = [&](int i) { return V.f(f_pi, i, jj, kk) * V.t(T_SQRTG, i, jj); };
-      const double t1 = d_r(fr, ii, ni, G.rr) / sg;
-      const double t2 = d_th(ft, jj, kk, nj, kt, G.rt) / sg;
-      const double t3 = EF ? d_ph(fp, kk, Nz, G.rp) : d_ph(fp, kk, Nz, G.rp) / sg;
-      const double t4 = Pi * dxb;
-      const double t5 = be * a_r(ps, ii, ni, be, G.rr) / sg;
-      const double t6 = am2 * V.f(f_phi, ii, jj, kk);
-      R[4] = t1 + t2 + t3 + t4 + t5 - t6;
-    }
+  const double rr = G.rr, rt = G.rt, rp = G.rp, dt = G.dt;
+  const int c = lane + HALO_R;  // own column in the r-arrays
+  const uint32_t sysb = (uint32_t)(sys * 8);  // outputs: system offset in bytes
 
-    const int64_t p = (int64_t)kk * G.plane + (int64_t)jj * G.ld + ii + sys;
-    if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
-#pragma unroll
-      for (int c = 0; c < 5; ++c) {
-        const double k = R[c] * dt;
-        double acc;
-        if constexpr (MODE == M_RK4_S1)
-          acc = k;
-        else
-          acc = a.aux[c][p] + 2.0 * k;
-        a.out2[c][p] = acc;
-        if constexpr (MODE == M_RK4_S3)
-          a.out[c][p] = a.base[c][p] + k;
-        else
-          a.out[c][p] = a.base[c][p] + 0.50 * k;
+  for (int kk = k0; kk < k1; ++kk) {
+    const uint32_t sk = (uint32_t)kk * plane8;
+    // --- products of plane kk into LDS (from registers) ----------------------
+    if (valid) {
+      if constexpr (N14) {
+        sG[row][c] = al * Pi;
+        sPR[row][c] = psr;
+        sF[row + 1][lane] = F0;
       }
-    } else if constexpr (MODE == M_RK4_S4) {
-#pragma unroll
-      for (int c = 0; c < 5; ++c) {
-        const double k = R[c] * dt;
-        a.out[c][p] = a.base[c][p] + (a.aux[c][p] + k) / 6.0;
+      if constexpr (N5) {
+        if constexpr (EF)
+          sFR[row][c] = asg * grr * psr;
+        else
+          sFR[row][c] = asg * (grr * psr + grp * psp);
+        sPS[row][c] = Pi * sg;
+        sFT[row + 1][lane] = asgtt * pst;
       }
-    } else if constexpr (MODE == M_RK3_S1) {
-#pragma unroll
-      for (int c = 0; c < 5; ++c) a.out[c][p] = a.base[c][p] + R[c] * dt;
-    } else if constexpr (MODE == M_RK3_S2) {
-#pragma unroll
-      for (int c = 0; c < 5; ++c) a.out[c][p] = 0.750 * a.base[c][p] + 0.250 * (a.in[c][p] + R[c] * dt);
-    } else if constexpr (MODE == M_RK3_S3) {
-#pragma unroll
-      for (int c = 0; c < 5; ++c) a.out[c][p] = a.base[c][p] / 3.0 + 2.0 * (a.in[c][p] + R[c] * dt) / 3.0;
-    } else if constexpr (MODE == M_PIRK_1A) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) a.out[c][p] = a.base[c][p] + dt * R[c];
-    } else if constexpr (MODE == M_PIRK_1B) {
-      a.out[4][p] = a.base[4][p] + dt * R[4];
-    } else if constexpr (MODE == M_PIRK2_2) {
-#pragma unroll
-      for (int c = 0; c < 5; ++c) a.out[c][p] = 0.5 * (a.base[c][p] + a.in[c][p] + dt * R[c]);
-    } else if constexpr (MODE == M_PIRK3_2A) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) a.out[c][p] = 0.75 * a.base[c][p] + 0.25 * (a.aux[c][p] + dt * R[c]);
-    } else if constexpr (MODE == M_PIRK3_2B) {
-      a.out[4][p] = 0.75 * a.base[4][p] + 0.25 * (a.aux[4][p] + dt * R[4]);
-    } else if constexpr (MODE == M_PIRK3_3) {
-      const double third = 1.0 / 3.0, two_thirds = 2.0 / 3.0;
-#pragma unroll
-      for (int c = 0; c < 5; ++c) a.out[c][p] = third * a.base[c][p] + two_thirds * (a.in[c][p] + dt * R[c]);
     }
+    if (hkind == 1) {
+      if constexpr (N14) sF[hrow][hcol] = h0 * hx0 + h1 * hx1;
+      if constexpr (N5) sFT[hrow][hcol] = h2 * hx2;
+    } else if (hkind == 2) {
+      if constexpr (N14) {
+        sG[hrow][hcol] = h0 * hx0;
+        sPR[hrow][hcol] = hx1;
+      }
+      if constexpr (N5) {
+        if constexpr (EF)
+          sFR[hrow][hcol] = h1 * h2 * hx1;
+        else
+          sFR[hrow][hcol] = h1 * (h2 * hx1 + h3 * hx2);
+        sPS[hrow][hcol] = hx0 * h4;
+      }
+    }
+    // --- loads issued before the barrier: next plane, combine operands --------
+    double nPi = 0, npsr = 0, npst = 0, npsp = 0;
+    if (valid && kk < Nz) {
+      const uint32_t sn = sk + plane8 + own;
+      nPi = gld(f_pi, sn);
+      npsr = gld(f_psr, sn);
+      if constexpr (N5) {
+        npst = gld(f_pst, sn);
+        npsp = gld(f_psp, sn);
+      }
+    }
+    double nh0 = 0, nh1 = 0, nh2 = 0;
+    if (kk + 1 < k1) load_halo(sk + plane8, nh0, nh1, nh2);
+    const uint32_t so = sk + sysb + own;
+    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, vphi = 0;
+    if (act) {
+#pragma unroll
+      for (int f = CLO; f < CHI; ++f) vb[f] = gld(a.base[f], so);
+      if constexpr (CAUX) {
+#pragma unroll
+        for (int f = CLO; f < CHI; ++f) va[f] = gld(a.aux[f], so);
+      }
+      if constexpr (N5) vphi = gld(f_phi, sk + own);
+    }
+    __syncthreads();
+
+    if (act) {
+      const int kt = (kk + dis) % (Nz - 1);
+      const double Fp = Fof(nPi, npsr);
+      double fpp = 0;
+      if constexpr (N5) fpp = fpof(npsr, npsp);
+      double R[5] = {0, 0, 0, 0, 0};
+      // D_r / A_r on an LDS r-array (finite_differences.f90:36-42, :779-799)
+      auto Dr = [&](const double(*A)[W]) {
+        if (ii == 0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
+        if (ii == ni - 1) return (+3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+        return (-A[row][c - 1] + A[row][c + 1]) * rr;
+      };
+      auto Ar = [&](const double(*A)[W]) {
+        if (ii <= 1) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
+        if (ii == ni - 2) return (A[row][c + 1] - A[row][c - 1]) * rr;
+        if (ii == ni - 1) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+        if (be > 0.0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
+        if (be < 0.0) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+        return (A[row][c + 1] - A[row][c - 1]) * rr;
+      };
+      // D_theta on an LDS theta-array (finite_differences.f90:214-230); the
+      // pole row reads plane k_target and the last row may reach below the
+      // tile, both from global memory through `far`.
+      auto Dth = [&](const double(*A)[TI], auto far) {
+        if (jj == 0) return (-A[row + 2][lane] + far(2, kt)) * rt;
+        if (jj == nj - 1) {
+          const double m2 = row >= 1 ? A[row - 1][lane] : far(nj - 3, kk);
+          return (+3.0 * A[row + 1][lane] - 4.0 * A[row][lane] + m2) * rt;
+        }
+        return (-A[row][lane] + A[row + 2][lane]) * rt;
+      };
+      // D_phi (finite_differences.f90:328-335); k = Nz pairs f(Nz-1) with f(2)
+      auto Dph = [&](double qm, double qp, auto at2) {
+        if (kk == Nz) return (-at2() + qm) * rp;
+        return (-qm + qp) * rp;
+      };
+      if constexpr (N14) {
+        R[0] = F0;  // rhs_phi = alpha*Pi + beta*psi_r (:675)
+        R[1] = Dr(sG) + be * Ar(sPR) + dxb * psr;  // :687-689
+        R[2] = Dth(sF, [&](int j, int k) {
+          const uint32_t o = (uint32_t)((j * ld + ii) * 8), q = o + (uint32_t)k * plane8;
+          return T(T_ALPHA, o) * gld(f_pi, q) + T(T_BETA, o) * gld(f_psr, q);
+        });
+        R[3] = Dph(Fm, Fp, [&]() { return Fof(gld(f_pi, 2 * plane8 + own), gld(f_psr, 2 * plane8 + own)); });
+      }
+      if constexpr (N5) {
+        const double t1 = Dr(sFR) / sg;
+        const double t2 = Dth(sFT, [&](int j, int k) {
+                            const uint32_t o = (uint32_t)((j * ld + ii) * 8);
+                            return T(T_ASGTT, o) * gld(f_pst, o + (uint32_t)k * plane8);
+                          }) / sg;
+        const double dp =
+            Dph(fpm, fpp, [&]() { return fpof(gld(f_psr, 2 * plane8 + own), gld(f_psp, 2 * plane8 + own)); });
+        const double t3 = EF ? dp : dp / sg;
+        const double t4 = Pi * dxb;
+        const double t5 = be * Ar(sPS) / sg;
+        const double t6 = am2 * vphi;
+        R[4] = t1 + t2 + t3 + t4 + t5 - t6;  // :728-736
+      }
+      // stage combine; `cur` is the RHS input at this point
+      const double cur[5] = {vphi, psr, pst, psp, Pi};
+      if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          const double k = R[f] * dt;
+          double acc;
+          if constexpr (MODE == M_RK4_S1)
+            acc = k;
+          else
+            acc = va[f] + 2.0 * k;
+          gst(a.out2[f], so, acc);
+          if constexpr (MODE == M_RK4_S3)
+            gst(a.out[f], so, vb[f] + k);
+          else
+            gst(a.out[f], so, vb[f] + 0.50 * k);
+        }
+      } else if constexpr (MODE == M_RK4_S4) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(a.out[f], so, vb[f] + (va[f] + R[f] * dt) / 6.0);
+      } else if constexpr (MODE == M_RK3_S1) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(a.out[f], so, vb[f] + R[f] * dt);
+      } else if constexpr (MODE == M_RK3_S2) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(a.out[f], so, 0.750 * vb[f] + 0.250 * (cur[f] + R[f] * dt));
+      } else if constexpr (MODE == M_RK3_S3) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(a.out[f], so, vb[f] / 3.0 + 2.0 * (cur[f] + R[f] * dt) / 3.0);
+      } else if constexpr (MODE == M_PIRK_1A) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) gst(a.out[f], so, vb[f] + dt * R[f]);
+      } else if constexpr (MODE == M_PIRK_1B) {
+        gst(a.out[4], so, vb[4] + dt * R[4]);
+      } else if constexpr (MODE == M_PIRK2_2) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(a.out[f], so, 0.5 * (vb[f] + cur[f] + dt * R[f]));
+      } else if constexpr (MODE == M_PIRK3_2A) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) gst(a.out[f], so, 0.75 * vb[f] + 0.25 * (va[f] + dt * R[f]));
+      } else if constexpr (MODE == M_PIRK3_2B) {
+        gst(a.out[4], so, 0.75 * vb[4] + 0.25 * (va[4] + dt * R[4]));
+      } else if constexpr (MODE == M_PIRK3_3) {
+        const double third = 1.0 / 3.0, two_thirds = 2.0 / 3.0;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(a.out[f], so, third * vb[f] + two_thirds * (cur[f] + dt * R[f]));
+      }
+      Fm = F0;
+      fpm = fp0;
+      F0 = Fp;
+      fp0 = fpp;
+    } else {
+      Fm = F0;
+      fpm = fp0;
+      F0 = Fof(nPi, npsr);
+      if constexpr (N5) fp0 = fpof(npsr, npsp);
+    }
+    __syncthreads();
+    Pi = nPi;
+    psr = npsr;
+    pst = npst;
+    psp = npsp;
+    hx0 = nh0;
+    hx1 = nh1;
+    hx2 = nh2;
   }
 }
 
@@ -293,15 +470,20 @@ __global__ __launch_bounds__(256) void pack_kernel(const Geom G, double* __restr
     src[dense] = dst[c][pitched];
 }
 
-template <int MODE>
-hipError_t launch_mode(const StageArgs& a, hipStream_t s, int nsys) {
+template <int MODE, int TJ>
+hipError_t launch_mode_tj(const StageArgs& a, hipStream_t s, int nsys) {
   dim3 grid((unsigned)(a.nti * a.ntj * a.ntk), (unsigned)nsys, 1);
   dim3 block(TI, TJ, 1);
   if (a.G.metric_ef)
-    hipLaunchKernelGGL((stage_kernel<MODE, 1>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<MODE, 1, TJ>), grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL((stage_kernel<MODE, 0>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<MODE, 0, TJ>), grid, block, 0, s, a);
   return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_mode(const StageArgs& a, hipStream_t s, int nsys) {
+  return a.tj == 8 ? launch_mode_tj<MODE, 8>(a, s, nsys) : launch_mode_tj<MODE, 4>(a, s, nsys);
 }
 
 }  // namespace
@@ -309,8 +491,10 @@ hipError_t launch_mode(const StageArgs& a, hipStream_t s, int nsys) {
 hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
   StageArgs a = a0;
   const int nsys = a.nsys;
+  if (a.tj != 4 && a.tj != 8) a.tj = 4;
+  if (a.kc < 1) a.kc = 1;
   a.nti = (a.i_hi - a.i_lo + TI - 1) / TI;
-  a.ntj = (a.j_hi - a.j_lo + TJ - 1) / TJ;
+  a.ntj = (a.j_hi - a.j_lo + a.tj - 1) / a.tj;
   a.ntk = (a.k_hi - a.k_lo + a.kc - 1) / a.kc;
   switch (mode) {
     case M_RK4_S1: return launch_mode<M_RK4_S1>(a, s, nsys);

@@ -180,7 +180,11 @@ class LinesMethod:
     def sync(self) -> None:
         capi.check(self._lib.kg_sync(self._h), self._h)
 
-    # -- timing -----------------------------------------------------------------
+    # -- launch shape and timing -------------------------------------------------
+    def set_tuning(self, kc: int, tj: int) -> None:
+        """phi planes per workgroup and theta rows per workgroup of the stage kernel."""
+        capi.check(self._lib.kg_set_tuning(self._h, int(kc), int(tj)), self._h)
+
     def profile(self, on: bool) -> None:
         capi.check(self._lib.kg_profile_enable(self._h, 1 if on else 0), self._h)
 

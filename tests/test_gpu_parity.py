@@ -139,3 +139,28 @@ def test_config1_size_pirk2_and_rk4():
         assert max(_rel_err(got, sim.u)) <= 1e-13, scheme
         assert np.array_equal(got, sim.u), scheme
         gpu.close()
+
+
+@pytest.mark.parametrize("kc,tj", [(1, 4), (3, 8), (5, 4), (64, 8)])
+@pytest.mark.parametrize("scheme", ["pirk3", "rk4"])
+def test_launch_shape_does_not_change_results(kc, tj, scheme):
+    p = CASES[1][1]
+    sim, gpu, _ = _pair(p, scheme, seed=21)
+    gpu.set_tuning(kc, tj)
+    sim.step(scheme, 5)
+    gpu.step(5)
+    assert np.array_equal(gpu.u_soa()[0], sim.u), (kc, tj)
+
+
+def test_autotune_restores_state():
+    import kg_amd
+    from kg_amd.autotune import autotune
+    p = CASES[0][1]
+    sim, gpu, u0 = _pair(p, "pirk2", seed=8)
+    gpu.step(3)
+    sim.step("pirk2", 3)
+    best, timings = autotune(gpu, candidates=[(2, 4), (8, 8)], steps=1)
+    assert best in timings
+    gpu.step(2)
+    sim.step("pirk2", 2)
+    assert np.array_equal(gpu.u_soa()[0], sim.u)
