@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,7 @@ thread_local std::string g_last_error;
 
 struct State5 {
   double* f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* alloc = nullptr;  // slot 0 of the buffer the fields live in
 };
 
 }  // namespace
@@ -47,8 +49,10 @@ struct kg_handle {
   double* S5[2] = {nullptr, nullptr};
   State5 S2;            // MoL%u_stage_2
   double* S25[2] = {nullptr, nullptr};
-  State5 Ua, Ub, A;     // RK stage states and the RK4 accumulator
+  State5 Ua, Ub, A;     // RK3 stage states; RK4 k2, k3 and the buffer that
+                        // alternates with u as (k1, next state)
   int p5 = 0, p25 = 0;  // slot holding the current u_stage(5), u_stage_2(5)
+  int px = 0;           // RK4: which of the two state buffers holds u
   int tune_kc = 32;     // phi planes per workgroup of the stage kernel
   int tune_tj = 4;      // theta rows per workgroup
   double t = 0.0;
@@ -92,6 +96,7 @@ kg_status alloc_state(kg_handle* h, State5* s, int nfields_extra, double** extra
   h->allocs.push_back(p);
   KG_HIP(h, hipMemsetAsync(p, 0, n * sizeof(double), h->stream));
   double* d = (double*)p;
+  s->alloc = d;
   for (int c = 0; c < 5; ++c) s->f[c] = d + c * per;
   for (int x = 0; x < nfields_extra; ++x) extra[x] = d + (5 + x) * per;
   return KG_OK;
@@ -152,17 +157,24 @@ Box full_box(const Geom& G) { return Box{0, G.ni, 0, G.nj, 0, G.nk}; }
 // i in [-g+1, Nx+g-1], j in [1, Ny-1], k in [0, Nz-1].
 Box filled_box(const Geom& G) { return Box{1, G.ni - 1, G.g + 1, G.g + G.Ny, 0, G.Nz}; }
 
-kg_status stage(kg_handle* h, int mode, const Box& b, const State5& in, const State5* base, const State5* aux,
-                const State5* out, const State5* out2) {
+struct Roles {
+  const State5* r[R_NUM] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+kg_status stage(kg_handle* h, int mode, const Box& b, const Roles& roles) {
   StageArgs a{};
   a.G = h->G;
   a.tab = h->d_tab;
-  for (int c = 0; c < 5; ++c) {
-    a.in[c] = in.f[c];
-    a.base[c] = base ? base->f[c] : nullptr;
-    a.aux[c] = aux ? aux->f[c] : nullptr;
-    a.out[c] = out ? out->f[c] : nullptr;
-    a.out2[c] = out2 ? out2->f[c] : nullptr;
+  a.per = (int64_t)h->G.sys_stride * h->nsys;
+  for (int r = 0; r < R_NUM; ++r) {
+    const State5* st = roles.r[r];
+    a.rb[r] = st ? st->alloc : nullptr;
+    a.ri4[r] = 4;
+    if (st && st->f[4]) {
+      const int64_t slot = (st->f[4] - st->alloc) / a.per;
+      if (slot != 4 && slot != 5) return fail(h, KG_ERR_INVALID, "internal: Pi slot %lld", (long long)slot);
+      a.ri4[r] = (int)slot;
+    }
   }
   a.i_lo = b.i_lo;
   a.i_hi = b.i_hi;
@@ -186,6 +198,18 @@ kg_status stage(kg_handle* h, int mode, const Box& b, const State5& in, const St
     h->ev_marks.push_back({0, h->ev_used - 2});
   }
   return KG_OK;
+}
+
+Roles roles(const State5* in, const State5* base, const State5* out, const State5* aux = nullptr,
+            const State5* kin = nullptr, const State5* aux2 = nullptr) {
+  Roles r;
+  r.r[R_IN] = in;
+  r.r[R_BASE] = base;
+  r.r[R_OUT] = out;
+  r.r[R_AUX] = aux;
+  r.r[R_KIN] = kin;
+  r.r[R_AUX2] = aux2;
+  return r;
 }
 
 kg_status fill(kg_handle* h, const State5& u, const State5& u_p) {
@@ -216,27 +240,29 @@ kg_status fill(kg_handle* h, const State5& u, const State5& u_p) {
     if (s_ != KG_OK) return s_;   \
   } while (0)
 
-kg_status enqueue_rk4(kg_handle* h) {
-  const Geom& G = h->G;
-  const Box fb = filled_box(G);
-  KG_TRY(stage(h, M_RK4_S1, fb, h->u, &h->u, nullptr, &h->Ua, &h->A));
-  KG_TRY(fill(h, h->Ua, h->u));
-  KG_TRY(stage(h, M_RK4_S2, fb, h->Ua, &h->u, &h->A, &h->Ub, &h->A));
-  KG_TRY(fill(h, h->Ub, h->u));
-  KG_TRY(stage(h, M_RK4_S3, fb, h->Ub, &h->u, &h->A, &h->Ua, &h->A));
-  KG_TRY(fill(h, h->Ua, h->u));
-  KG_TRY(stage(h, M_RK4_S4, fb, h->Ua, &h->u, &h->A, &h->u, nullptr));
-  KG_TRY(fill(h, h->u, h->u));
+// rk4 (lines_method_lib.f90:424-446) with the increments k_s = rhs*dt kept
+// instead of the stage states: stage s reads u_p + c*k_{s-1} (and its ghost
+// fill) where it needs it, which is the same rounded value the reference
+// stores in u.  The new state is written over k1 (read only at its own
+// point in the last stage), and u and that buffer trade places each step.
+kg_status enqueue_rk4(kg_handle* h, const State5& P, const State5& X) {
+  const Box fb = filled_box(h->G);
+  const State5& K1 = X;
+  KG_TRY(stage(h, M_RK4_S1, fb, roles(&P, nullptr, &K1)));
+  KG_TRY(stage(h, M_RK4_S2, fb, roles(&P, nullptr, &h->Ua, nullptr, &K1)));
+  KG_TRY(stage(h, M_RK4_S3, fb, roles(&P, nullptr, &h->Ub, nullptr, &h->Ua)));
+  KG_TRY(stage(h, M_RK4_S4, fb, roles(&P, &P, &X, &K1, &h->Ub, &h->Ua)));
+  KG_TRY(fill(h, X, P));
   return KG_OK;
 }
 
 kg_status enqueue_rk3(kg_handle* h) {
   const Box fb = filled_box(h->G);
-  KG_TRY(stage(h, M_RK3_S1, fb, h->u, &h->u, nullptr, &h->Ua, nullptr));
+  KG_TRY(stage(h, M_RK3_S1, fb, roles(&h->u, &h->u, &h->Ua)));
   KG_TRY(fill(h, h->Ua, h->u));
-  KG_TRY(stage(h, M_RK3_S2, fb, h->Ua, &h->u, nullptr, &h->Ub, nullptr));
+  KG_TRY(stage(h, M_RK3_S2, fb, roles(&h->Ua, &h->u, &h->Ub)));
   KG_TRY(fill(h, h->Ub, h->u));
-  KG_TRY(stage(h, M_RK3_S3, fb, h->Ub, &h->u, nullptr, &h->u, nullptr));
+  KG_TRY(stage(h, M_RK3_S3, fb, roles(&h->Ub, &h->u, &h->u)));
   KG_TRY(fill(h, h->u, h->u));
   return KG_OK;
 }
@@ -245,13 +271,12 @@ kg_status enqueue_rk3(kg_handle* h) {
 // previous step's S(5) held in slot p5.  Leaves the new S(5) in slot 1-p5.
 kg_status enqueue_pirk_stage1(kg_handle* h, int p5) {
   const Box all = full_box(h->G);
-  State5 s_out = h->S;
-  KG_TRY(stage(h, M_PIRK_1A, all, h->u, &h->u, nullptr, &s_out, nullptr));
+  KG_TRY(stage(h, M_PIRK_1A, all, roles(&h->u, &h->u, &h->S)));
   State5 in = h->S;
   in.f[4] = h->S5[p5];
-  State5 out{};
+  State5 out = h->S;
   out.f[4] = h->S5[1 - p5];
-  KG_TRY(stage(h, M_PIRK_1B, all, in, &h->u, nullptr, &out, nullptr));
+  KG_TRY(stage(h, M_PIRK_1B, all, roles(&in, &h->u, &out)));
   return KG_OK;
 }
 
@@ -260,7 +285,7 @@ kg_status enqueue_pirk2(kg_handle* h, int p5) {
   State5 s = h->S;
   s.f[4] = h->S5[1 - p5];
   // :294-304 u = 0.5*(u_p + u_stage + dt*rhs(u_stage)), then boundary (:306)
-  KG_TRY(stage(h, M_PIRK2_2, filled_box(h->G), s, &h->u, nullptr, &h->u, nullptr));
+  KG_TRY(stage(h, M_PIRK2_2, filled_box(h->G), roles(&s, &h->u, &h->u)));
   KG_TRY(fill(h, h->u, h->u));
   return KG_OK;
 }
@@ -271,25 +296,24 @@ kg_status enqueue_pirk3(kg_handle* h, int p5, int p25) {
   s.f[4] = h->S5[1 - p5];
   const Box all = full_box(h->G);
   // :356-364 u_stage_2(1..4)
-  State5 s2out = h->S2;
-  KG_TRY(stage(h, M_PIRK3_2A, all, s, &h->u, &s, &s2out, nullptr));
+  KG_TRY(stage(h, M_PIRK3_2A, all, roles(&s, &h->u, &h->S2, &s)));
   // :366-367 u_stage_2(5) from rhs_pi(u_stage_2) with the previous u_stage_2(5)
   State5 in2 = h->S2;
   in2.f[4] = h->S25[p25];
-  State5 out2{};
+  State5 out2 = h->S2;
   out2.f[4] = h->S25[1 - p25];
-  KG_TRY(stage(h, M_PIRK3_2B, all, in2, &h->u, &s, &out2, nullptr));
+  KG_TRY(stage(h, M_PIRK3_2B, all, roles(&in2, &h->u, &out2, &s)));
   // :372-384
   State5 s2 = h->S2;
   s2.f[4] = h->S25[1 - p25];
-  KG_TRY(stage(h, M_PIRK3_3, filled_box(h->G), s2, &h->u, nullptr, &h->u, nullptr));
+  KG_TRY(stage(h, M_PIRK3_3, filled_box(h->G), roles(&s2, &h->u, &h->u)));
   KG_TRY(fill(h, h->u, h->u));
   return KG_OK;
 }
 
 kg_status enqueue_step(kg_handle* h, int scheme) {
   switch (scheme) {
-    case KG_RK4: return enqueue_rk4(h);
+    case KG_RK4: return enqueue_rk4(h, h->u, h->A);
     case KG_RK3: return enqueue_rk3(h);
     case KG_PIRK2: return enqueue_pirk2(h, h->p5);
     case KG_PIRK3: return enqueue_pirk3(h, h->p5, h->p25);
@@ -300,6 +324,10 @@ kg_status enqueue_step(kg_handle* h, int scheme) {
 void advance_parity(kg_handle* h, int scheme) {
   if (scheme == KG_PIRK2 || scheme == KG_PIRK3) h->p5 ^= 1;
   if (scheme == KG_PIRK3) h->p25 ^= 1;
+  if (scheme == KG_RK4) {
+    std::swap(h->u, h->A);  // the new state was written into the other buffer
+    h->px ^= 1;
+  }
 }
 
 kg_status ensure_buffers(kg_handle* h, int scheme) {
@@ -317,7 +345,7 @@ kg_status do_steps(kg_handle* h, int scheme, int nsteps) {
     if (h->prof) {
       KG_TRY(enqueue_step(h, scheme));
     } else {
-      const int key = scheme * 4 + h->p5 * 2 + h->p25;
+      const int key = scheme * 8 + h->px * 4 + h->p5 * 2 + h->p25;
       auto itg = h->graphs.find(key);
       if (itg == h->graphs.end()) {
         hipGraph_t graph;

@@ -55,10 +55,12 @@ int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m,
 
 // Integrator stage kinds (one kernel launch each).
 enum StageMode : int {
-  M_RK4_S1 = 0,   // k = R(P)dt; A = k; U = P + 0.5k
-  M_RK4_S2 = 1,   // k = R(U)dt; A = A + 2k; U' = P + 0.5k
-  M_RK4_S3 = 2,   // k = R(U)dt; A = A + 2k; U' = P + k
-  M_RK4_S4 = 3,   // k = R(U)dt; P = P + (A + k)/6
+  // RK4 keeps the increments k_s = R*dt and forms each stage state
+  // u = u_p + c*k_s (and its ghost fill) where it reads it
+  M_RK4_S1 = 0,   // k1 = R(P)dt
+  M_RK4_S2 = 1,   // k2 = R(P + 0.5k1)dt
+  M_RK4_S3 = 2,   // k3 = R(P + 0.5k2)dt
+  M_RK4_S4 = 3,   // X = P + (((k1 + 2k2) + 2k3) + R(P + k3)dt)/6
   M_RK3_S1 = 4,   // U1 = P + R(P)dt
   M_RK3_S2 = 5,   // U2 = 0.75P + 0.25(U1 + R(U1)dt)
   M_RK3_S3 = 6,   // P = P/3 + 2(U2 + R(U2)dt)/3
@@ -71,14 +73,26 @@ enum StageMode : int {
   M_NUM_MODES = 13
 };
 
+// Roles of the buffers a stage reads and writes.  Every state buffer holds
+// 5 (or 6) field slots `per` doubles apart, so a role is its base address;
+// field 4 (Pi) may sit in slot 4 or in the extra slot 5 (the PIRK stage
+// arrays keep two Pi slots).
+enum Role : int {
+  R_IN = 0,    // fields the RHS is evaluated on (RK4 stages 2-4: u_p)
+  R_KIN = 1,   // RK4 stages 2-4: increment k the stage state u_p + c*k is formed from
+  R_BASE = 2,  // u_p / P / u of the combine
+  R_AUX = 3,   // RK4 k1 (stage 4), PIRK3 stage state (stage 2)
+  R_AUX2 = 4,  // RK4 k2 (stage 4)
+  R_OUT = 5,   // stage output
+  R_NUM = 6
+};
+
 struct StageArgs {
   Geom G;
   const double* tab;        // 10 tables, each nj*ld
-  const double* in[5];      // fields the RHS is evaluated on
-  const double* base[5];    // u_p / P / u
-  const double* aux[5];     // A (RK4), S (PIRK3 stage 2), S (PIRK2 stage 2)
-  double* out[5];           // stage output
-  double* out2[5];          // RK4 accumulator output
+  double* rb[R_NUM];        // base of each role's buffer (slot 0)
+  int ri4[R_NUM];           // slot of field 4 in each role (4 or 5)
+  int64_t per;              // doubles between two field slots
   int i_lo, i_hi, j_lo, j_hi, k_lo, k_hi;  // computed box, [lo, hi)
   int kc;                   // phi planes per workgroup
   int tj;                   // theta rows per workgroup (4 or 8)

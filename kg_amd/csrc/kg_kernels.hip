@@ -63,20 +63,29 @@ __device__ __forceinline__ int remap_xcd(int b, int nb) {
 template <int MODE>
 struct Combine {
   // operands of the stage combine read at the point: base (u_p, P, u) and
-  // aux (RK4 accumulator, PIRK stage state); fields [lo, hi)
+  // aux (PIRK3 stage state; RK4 k1 and k2 in stage 4); fields [lo, hi)
   static constexpr int lo = (MODE == M_PIRK_1B || MODE == M_PIRK3_2B) ? 4 : 0;
   static constexpr int hi = (MODE == M_PIRK_1A || MODE == M_PIRK3_2A) ? 4 : 5;
-  static constexpr bool aux = MODE == M_RK4_S2 || MODE == M_RK4_S3 || MODE == M_RK4_S4 || MODE == M_PIRK3_2A ||
-                              MODE == M_PIRK3_2B;
-  static constexpr bool base = MODE != M_RK4_S1 || true;
+  static constexpr bool base = !(MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3);
+  static constexpr bool aux = MODE == M_RK4_S4 || MODE == M_PIRK3_2A || MODE == M_PIRK3_2B;
+  // RK4 stages 2-4 read the stage state u_p + c*k and its ghost fill
+  static constexpr bool virt = MODE == M_RK4_S2 || MODE == M_RK4_S3 || MODE == M_RK4_S4;
+  // stages followed by a ghost fill compute only i in [1, ni-2],
+  // j in [g+1, g+Ny-1], k in [0, Nz-1]: the edge rows of D_r at i = -g and
+  // Nx+g, of D_theta at j = -g and Ny+g and of D_phi at k = Nz never occur
+  static constexpr bool boxed = !(MODE == M_PIRK_1A || MODE == M_PIRK_1B || MODE == M_PIRK3_2A || MODE == M_PIRK3_2B);
 };
 
+// RK4 stage 4 holds the most operands; asking for 4 waves per SIMD (a few
+// spilled registers) measured faster than the 3 its natural allocation gives.
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kernel(const StageArgs a) {
   constexpr bool N14 = ModeTraits<MODE>::n14;
   constexpr bool N5 = ModeTraits<MODE>::n5;
   constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
-  constexpr bool CAUX = Combine<MODE>::aux;
+  constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
+  constexpr bool BOXED = Combine<MODE>::boxed;
+
   constexpr int W = TI + 2 * HALO_R;
   __shared__ double sG[TJ][W];    // alpha*Pi
   __shared__ double sPR[TJ][W];   // psi_r
@@ -95,7 +104,7 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
   const int tid = row * TI + lane;
   const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
   const int ii = i0 + lane, jj = j0 + row;
-  const int ni = G.ni, nj = G.nj, Nz = G.Nz, ld = G.ld;
+  const int ni = G.ni, nj = G.nj, Nz = G.Nz, Ny = G.Ny, g = G.g, ld = G.ld;
   const uint32_t plane8 = (uint32_t)(G.plane * 8);
   const uint32_t n2b = (uint32_t)((int64_t)nj * ld * 8);  // bytes of one table
   const bool valid = ii < ni && jj < nj;
@@ -103,13 +112,63 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
   const int k0 = a.k_lo + tk * a.kc;
   const int k1 = min(k0 + a.kc, a.k_hi);
   const int64_t sys = (int64_t)blockIdx.y * G.sys_stride;
-  const double* f_phi = a.in[0] + sys;
-  const double* f_psr = a.in[1] + sys;
-  const double* f_pst = a.in[2] + sys;
-  const double* f_psp = a.in[3] + sys;
-  const double* f_pi = a.in[4] + sys;
+  const uint32_t sysb = (uint32_t)(sys * 8);  // system offset in bytes
   const double* tab = a.tab;
   auto T = [&](int which, uint32_t off) { return gld(tab, off + (uint32_t)which * n2b); };
+  // address of field f of a role: slot f (field 4: slot ri4) of that role's buffer
+  auto FP = [&](int r, int f) { return a.rb[r] + (int64_t)(f == 4 ? a.ri4[r] : f) * a.per; };
+  auto OFF = [&](int i, int j, int k) { return (uint32_t)((j * ld + i) * 8) + (uint32_t)k * plane8; };
+  const int dis = Nz / 2;
+
+  // Value of field f of the stage input at padded point (i, j, k).  For the
+  // RK4 stages that form u = u_p + c*k, points the ghost fill writes are
+  // mapped to the evolved points they copy (lines_method_lib.f90:313-330,
+  // 516-596): zL/zR, then yL/yR at k_target, then xL (extrapolation) / xR.
+  // remap() resolves a point once; Ur() reads any field there.
+  auto E = [&](int f, uint32_t o) {
+    if constexpr (VIRT) {
+      const double p = gld(FP(R_IN, f), o + sysb), k = gld(FP(R_KIN, f), o + sysb);
+      if constexpr (MODE == M_RK4_S4)
+        return p + k;  // u_p + k3
+      else
+        return p + 0.50 * k;  // u_p + 0.5*k1, u_p + 0.5*k2
+    } else {
+      return gld(FP(R_IN, f), o + sysb);
+    }
+  };
+  struct Src {
+    uint32_t o;
+    int kind;  // 0: evolved value at o; 1: xL extrapolation from o, o+1, o+2; 2: u_p at o (inflow)
+  };
+  auto remap = [&](int i, int j, int k) -> Src {
+    if constexpr (VIRT) {
+      if (i < 1 || i > ni - 2 || j < g + 1 || j > g + Ny - 1 || k < 1 || k > Nz - 1) {
+        if (k == 0)
+          k = Nz - 1;
+        else if (k == Nz)
+          k = 1;
+        if (j <= g) {
+          j = 2 * g + 1 - j;
+          k = (k + dis) % (Nz - 1);
+        } else if (j >= g + Ny) {
+          j = 2 * Ny + 2 * g - 1 - j;
+          k = (k + dis) % (Nz - 1);
+        }
+        if (i == 0) return Src{OFF(1, j, k), 1};
+        if (i == ni - 1) return G.bc_inflow ? Src{OFF(ni - 1, j, k), 2} : Src{OFF(ni - 2, j, k), 0};
+        return Src{OFF(i, j, k), 0};
+      }
+    }
+    return Src{OFF(i, j, k), 0};
+  };
+  auto Ur = [&](int f, const Src& q) -> double {
+    if constexpr (VIRT) {
+      if (q.kind == 1) return 3.0 * E(f, q.o) - 3.0 * E(f, q.o + 8) + E(f, q.o + 16);
+      if (q.kind == 2) return gld(FP(R_IN, f), q.o + sysb);
+    }
+    return E(f, q.o);
+  };
+  auto U = [&](int f, int i, int j, int k) { return Ur(f, remap(i, j, k)); };
 
   // own coefficients (the theta-ghost row of the tables is already the copy)
   const uint32_t own = valid ? (uint32_t)((jj * ld + ii) * 8) : 0u;
@@ -160,18 +219,20 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
     h3 = T(T_GRP, hown);
     h4 = T(T_SQRTG, hown);
   }
-  // halo raw values: Pi, psi_r and psi_theta (theta rows) or psi_phi (r columns)
-  auto load_halo = [&](uint32_t sk, double& x0, double& x1, double& x2) {
+  // halo values: Pi, psi_r and psi_theta (theta rows) or psi_phi (r columns)
+  auto load_halo = [&](int k, double& x0, double& x1, double& x2) {
+    if (hkind == 0) return;
+    const Src q = remap(hi, hj, k);
     if (hkind == 1) {
       if constexpr (N14) {
-        x0 = gld(f_pi, hown + sk);
-        x1 = gld(f_psr, hown + sk);
+        x0 = Ur(4, q);
+        x1 = Ur(1, q);
       }
-      if constexpr (N5) x2 = gld(f_pst, hown + sk);
-    } else if (hkind == 2) {
-      x0 = gld(f_pi, hown + sk);
-      x1 = gld(f_psr, hown + sk);
-      if constexpr (N5 && !EF) x2 = gld(f_psp, hown + sk);
+      if constexpr (N5) x2 = Ur(2, q);
+    } else {
+      x0 = Ur(4, q);
+      x1 = Ur(1, q);
+      if constexpr (N5 && !EF) x2 = Ur(3, q);
     }
   };
 
@@ -181,25 +242,24 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
   double hx0 = 0, hx1 = 0, hx2 = 0;  // halo values at plane kk
   if (valid) {
     const int km = k0 == 0 ? Nz - 2 : k0 - 1;  // D_phi at k = 0 pairs f(Nz-2) with f(1)
-    const uint32_t sm = (uint32_t)km * plane8 + own;
-    Fm = Fof(gld(f_pi, sm), gld(f_psr, sm));
-    if constexpr (N5) fpm = fpof(gld(f_psr, sm), gld(f_psp, sm));
-    const uint32_t s0 = (uint32_t)k0 * plane8 + own;
-    Pi = gld(f_pi, s0);
-    psr = gld(f_psr, s0);
+    const Src qm = remap(ii, jj, km);
+    const double mPi = Ur(4, qm), mpr = Ur(1, qm);
+    Fm = Fof(mPi, mpr);
+    if constexpr (N5) fpm = fpof(mpr, Ur(3, qm));
+    const Src q0 = remap(ii, jj, k0);
+    Pi = Ur(4, q0);
+    psr = Ur(1, q0);
     if constexpr (N5) {
-      pst = gld(f_pst, s0);
-      psp = gld(f_psp, s0);
+      pst = Ur(2, q0);
+      psp = Ur(3, q0);
       fp0 = fpof(psr, psp);
     }
     F0 = Fof(Pi, psr);
   }
-  load_halo((uint32_t)k0 * plane8, hx0, hx1, hx2);
+  load_halo(k0, hx0, hx1, hx2);
 
-  const int dis = Nz / 2;
   const double rr = G.rr, rt = G.rt, rp = G.rp, dt = G.dt;
   const int c = lane + HALO_R;  // own column in the r-arrays
-  const uint32_t sysb = (uint32_t)(sys * 8);  // outputs: system offset in bytes
 
   for (int kk = k0; kk < k1; ++kk) {
     const uint32_t sk = (uint32_t)kk * plane8;
@@ -238,30 +298,39 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
     // --- loads issued before the barrier: next plane, combine operands --------
     double nPi = 0, npsr = 0, npst = 0, npsp = 0;
     if (valid && kk < Nz) {
-      const uint32_t sn = sk + plane8 + own;
-      nPi = gld(f_pi, sn);
-      npsr = gld(f_psr, sn);
+      const Src qn = remap(ii, jj, kk + 1);
+      nPi = Ur(4, qn);
+      npsr = Ur(1, qn);
       if constexpr (N5) {
-        npst = gld(f_pst, sn);
-        npsp = gld(f_psp, sn);
+        npst = Ur(2, qn);
+        npsp = Ur(3, qn);
       }
     }
     double nh0 = 0, nh1 = 0, nh2 = 0;
-    if (kk + 1 < k1) load_halo(sk + plane8, nh0, nh1, nh2);
+    if (kk + 1 < k1) load_halo(kk + 1, nh0, nh1, nh2);
     const uint32_t so = sk + sysb + own;
-    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, vphi = 0;
-    if (act) {
-#pragma unroll
-      for (int f = CLO; f < CHI; ++f) vb[f] = gld(a.base[f], so);
-      if constexpr (CAUX) {
-#pragma unroll
-        for (int f = CLO; f < CHI; ++f) va[f] = gld(a.aux[f], so);
-      }
-      if constexpr (N5) vphi = gld(f_phi, sk + own);
-    }
     __syncthreads();
 
     if (act) {
+      // combine operands at the point (issued ahead of the RHS arithmetic)
+      double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+      if constexpr (CBASE) {
+#pragma unroll
+        for (int f = CLO; f < CHI; ++f) vb[f] = gld(FP(R_BASE, f), so);
+      }
+      if constexpr (CAUX) {
+#pragma unroll
+        for (int f = CLO; f < CHI; ++f) va[f] = gld(FP(R_AUX, f), so);
+      }
+      if constexpr (MODE == M_RK4_S4) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          va2[f] = gld(FP(R_AUX2, f), so);
+          vk[f] = gld(FP(R_KIN, f), so);
+        }
+      }
+      double vphi = 0;
+      if constexpr (N5) vphi = U(0, ii, jj, kk);
       const int kt = (kk + dis) % (Nz - 1);
       const double Fp = Fof(nPi, npsr);
       double fpp = 0;
@@ -269,14 +338,18 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
       double R[5] = {0, 0, 0, 0, 0};
       // D_r / A_r on an LDS r-array (finite_differences.f90:36-42, :779-799)
       auto Dr = [&](const double(*A)[W]) {
-        if (ii == 0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
-        if (ii == ni - 1) return (+3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+        if constexpr (!BOXED) {
+          if (ii == 0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
+          if (ii == ni - 1) return (+3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+        }
         return (-A[row][c - 1] + A[row][c + 1]) * rr;
       };
       auto Ar = [&](const double(*A)[W]) {
         if (ii <= 1) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
         if (ii == ni - 2) return (A[row][c + 1] - A[row][c - 1]) * rr;
-        if (ii == ni - 1) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+        if constexpr (!BOXED) {
+          if (ii == ni - 1) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+        }
         if (be > 0.0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
         if (be < 0.0) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
         return (A[row][c + 1] - A[row][c - 1]) * rr;
@@ -285,35 +358,37 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
       // pole row reads plane k_target and the last row may reach below the
       // tile, both from global memory through `far`.
       auto Dth = [&](const double(*A)[TI], auto far) {
-        if (jj == 0) return (-A[row + 2][lane] + far(2, kt)) * rt;
-        if (jj == nj - 1) {
-          const double m2 = row >= 1 ? A[row - 1][lane] : far(nj - 3, kk);
-          return (+3.0 * A[row + 1][lane] - 4.0 * A[row][lane] + m2) * rt;
+        if constexpr (!BOXED) {
+          if (jj == 0) return (-A[row + 2][lane] + far(2, kt)) * rt;
+          if (jj == nj - 1) {
+            const double m2 = row >= 1 ? A[row - 1][lane] : far(nj - 3, kk);
+            return (+3.0 * A[row + 1][lane] - 4.0 * A[row][lane] + m2) * rt;
+          }
         }
         return (-A[row][lane] + A[row + 2][lane]) * rt;
       };
       // D_phi (finite_differences.f90:328-335); k = Nz pairs f(Nz-1) with f(2)
       auto Dph = [&](double qm, double qp, auto at2) {
-        if (kk == Nz) return (-at2() + qm) * rp;
+        if constexpr (!BOXED) {
+          if (kk == Nz) return (-at2() + qm) * rp;
+        }
         return (-qm + qp) * rp;
       };
       if constexpr (N14) {
         R[0] = F0;  // rhs_phi = alpha*Pi + beta*psi_r (:675)
         R[1] = Dr(sG) + be * Ar(sPR) + dxb * psr;  // :687-689
         R[2] = Dth(sF, [&](int j, int k) {
-          const uint32_t o = (uint32_t)((j * ld + ii) * 8), q = o + (uint32_t)k * plane8;
-          return T(T_ALPHA, o) * gld(f_pi, q) + T(T_BETA, o) * gld(f_psr, q);
+          const uint32_t o = (uint32_t)((j * ld + ii) * 8);
+          return T(T_ALPHA, o) * U(4, ii, j, k) + T(T_BETA, o) * U(1, ii, j, k);
         });
-        R[3] = Dph(Fm, Fp, [&]() { return Fof(gld(f_pi, 2 * plane8 + own), gld(f_psr, 2 * plane8 + own)); });
+        R[3] = Dph(Fm, Fp, [&]() { return Fof(U(4, ii, jj, 2), U(1, ii, jj, 2)); });
       }
       if constexpr (N5) {
         const double t1 = Dr(sFR) / sg;
         const double t2 = Dth(sFT, [&](int j, int k) {
-                            const uint32_t o = (uint32_t)((j * ld + ii) * 8);
-                            return T(T_ASGTT, o) * gld(f_pst, o + (uint32_t)k * plane8);
+                            return T(T_ASGTT, (uint32_t)((j * ld + ii) * 8)) * U(2, ii, j, k);
                           }) / sg;
-        const double dp =
-            Dph(fpm, fpp, [&]() { return fpof(gld(f_psr, 2 * plane8 + own), gld(f_psp, 2 * plane8 + own)); });
+        const double dp = Dph(fpm, fpp, [&]() { return fpof(U(1, ii, jj, 2), U(3, ii, jj, 2)); });
         const double t3 = EF ? dp : dp / sg;
         const double t4 = Pi * dxb;
         const double t5 = be * Ar(sPS) / sg;
@@ -324,48 +399,37 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
       const double cur[5] = {vphi, psr, pst, psp, Pi};
       if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) {
-          const double k = R[f] * dt;
-          double acc;
-          if constexpr (MODE == M_RK4_S1)
-            acc = k;
-          else
-            acc = va[f] + 2.0 * k;
-          gst(a.out2[f], so, acc);
-          if constexpr (MODE == M_RK4_S3)
-            gst(a.out[f], so, vb[f] + k);
-          else
-            gst(a.out[f], so, vb[f] + 0.50 * k);
-        }
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, R[f] * dt);  // k = rhs_u*dt (:430,433,436)
       } else if constexpr (MODE == M_RK4_S4) {
+        // :439-440 u = u_p + (k1 + 2k2 + 2k3 + k4)/6, k4 = rhs_u*dt
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(a.out[f], so, vb[f] + (va[f] + R[f] * dt) / 6.0);
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * va2[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
       } else if constexpr (MODE == M_RK3_S1) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(a.out[f], so, vb[f] + R[f] * dt);
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + R[f] * dt);
       } else if constexpr (MODE == M_RK3_S2) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(a.out[f], so, 0.750 * vb[f] + 0.250 * (cur[f] + R[f] * dt));
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, 0.750 * vb[f] + 0.250 * (cur[f] + R[f] * dt));
       } else if constexpr (MODE == M_RK3_S3) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(a.out[f], so, vb[f] / 3.0 + 2.0 * (cur[f] + R[f] * dt) / 3.0);
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] / 3.0 + 2.0 * (cur[f] + R[f] * dt) / 3.0);
       } else if constexpr (MODE == M_PIRK_1A) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) gst(a.out[f], so, vb[f] + dt * R[f]);
+        for (int f = 0; f < 4; ++f) gst(FP(R_OUT, f), so, vb[f] + dt * R[f]);
       } else if constexpr (MODE == M_PIRK_1B) {
-        gst(a.out[4], so, vb[4] + dt * R[4]);
+        gst(FP(R_OUT, 4), so, vb[4] + dt * R[4]);
       } else if constexpr (MODE == M_PIRK2_2) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(a.out[f], so, 0.5 * (vb[f] + cur[f] + dt * R[f]));
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, 0.5 * (vb[f] + cur[f] + dt * R[f]));
       } else if constexpr (MODE == M_PIRK3_2A) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) gst(a.out[f], so, 0.75 * vb[f] + 0.25 * (va[f] + dt * R[f]));
+        for (int f = 0; f < 4; ++f) gst(FP(R_OUT, f), so, 0.75 * vb[f] + 0.25 * (va[f] + dt * R[f]));
       } else if constexpr (MODE == M_PIRK3_2B) {
-        gst(a.out[4], so, 0.75 * vb[4] + 0.25 * (va[4] + dt * R[4]));
+        gst(FP(R_OUT, 4), so, 0.75 * vb[4] + 0.25 * (va[4] + dt * R[4]));
       } else if constexpr (MODE == M_PIRK3_3) {
         const double third = 1.0 / 3.0, two_thirds = 2.0 / 3.0;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(a.out[f], so, third * vb[f] + two_thirds * (cur[f] + dt * R[f]));
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, third * vb[f] + two_thirds * (cur[f] + dt * R[f]));
       }
       Fm = F0;
       fpm = fp0;
@@ -382,6 +446,7 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(const StageArgs a) {
     psr = npsr;
     pst = npst;
     psp = npsp;
+
     hx0 = nh0;
     hx1 = nh1;
     hx2 = nh2;
