@@ -57,6 +57,47 @@ def cpu_baseline(max_seconds=20.0):
                       "8 phi planes), 1 thread, %.1f s" % (steps, Nx, Ny - 1, Nz - 1, el)}
 
 
+def batched_vs_plain(n, steps, tuning):
+    """The uneven batched workload: n evolutions (Nr in [256, 2048], (Ntheta, Nphi)
+    in {(16,32), (32,64), (64,128)}, m ~ U[0,0.5], M ~ U[0.5,2], RK4), advanced
+    `steps` steps (a) all together, one launch per stage (kg_batch_step), and
+    (b) one at a time through the plain path (kg_step graph replay per grid).
+    Aggregate grid-point updates/s of each; inputs resident in HBM."""
+    import torch
+    from kg_amd.batch import batch_step, make_member, uneven_specs
+
+    specs = uneven_specs(n)
+    members = []
+    for s in specs:
+        m = make_member(s, on_device=True)
+        m.set_tuning(*tuning)
+        members.append(m)
+    torch.cuda.empty_cache()
+    pts = sum(s.points for s in specs)
+    batch_step(members, 1)  # warm-up
+    members[0].sync()
+    t0 = time.perf_counter()
+    batch_step(members, steps)
+    members[0].sync()
+    tb = time.perf_counter() - t0
+    for m in members:  # warm-up (graph capture) of the plain path
+        m.step(1)
+        m.sync()
+    tp = 0.0
+    for m in members:
+        t0 = time.perf_counter()
+        m.step(steps)
+        m.sync()
+        tp += time.perf_counter() - t0
+    for m in members:
+        m.close()
+    vb, vp = pts * steps / tb, pts * steps / tp
+    return {"members": n, "steps": steps, "grid_points": pts, "batched_updates_per_s": vb,
+            "plain_updates_per_s": vp, "speedup": vb / vp,
+            "workload": "uneven RK4 batch: Nr~U{256..2048}, (Ntheta,Nphi) in {(16,32),(32,64),(64,128)}, "
+                        "m~U[0,0.5], M~U[0.5,2], seed 20261203"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +106,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kc", type=int, default=0, help="fix the stage kernel's phi planes per workgroup (skips autotune)")
     ap.add_argument("--tj", type=int, default=8, help="theta rows per workgroup when --kc is given")
+    ap.add_argument("--batched", type=int, default=256,
+                    help="members of the uneven batched workload to time beside the plain path (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (a single grid lives on one MI355X); "
@@ -74,15 +118,18 @@ def main():
     import numpy as np
     import torch
     import kg_amd
-    from kg_amd.initial_data import config2_state
+    from kg_amd.initial_data import random_lm_coeffs, shell_state_device
 
     assert torch.cuda.is_available(), "no GPU visible"
     print("bench: building initial data", file=sys.stderr, flush=True)
-    u0 = config2_state()
     lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
                             CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"])
-    lm.set_u_soa(u0[None])
+    u0 = shell_state_device(CFG["Nx"], CFG["Ny"], CFG["Nz"], 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0,
+                            random_lm_coeffs(20261203))
+    torch.cuda.synchronize()
+    lm.copy_state_device(u0.data_ptr(), to_handle=True)
     del u0
+    torch.cuda.empty_cache()
     lm.boundary()  # ghost-fill the initial data (the reference's IC fills x, y, z ghosts the same way)
     if args.kc > 0:
         best = (args.kc, args.tj)
@@ -144,6 +191,9 @@ def main():
                      "bytes_per_launch": per_launch_bytes},
         "fill_ms_per_step": fill_ms / args.steps,
     }
+    if args.batched > 0:
+        print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
+        out["batched"] = batched_vs_plain(args.batched, args.batched_steps, best)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

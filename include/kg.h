@@ -125,6 +125,14 @@ kg_status kg_rk4(kg_handle* h);
 /* nsteps steps of desc.scheme: the `do l=1,Nt` loop body of evolve.f90:22-26
  * without diagnostics, replayed from a captured HIP graph. */
 kg_status kg_step(kg_handle* h, int32_t nsteps);
+/* Batched stepping (no reference counterpart: the reference runs one grid
+ * per process).  Advances every handle of hs[0..n) by nsteps RK4 steps, each
+ * stage one kernel launch over the tiles of all grids (grids may differ in
+ * size, mass, black-hole parameters and initial data; all must be RK4 with
+ * the same metric form).  Work runs on hs[0]'s stream; later calls on any
+ * member are ordered after it.  Each member's result equals stepping it
+ * alone. */
+kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
 /* waits for the handle's stream */
 kg_status kg_sync(kg_handle* h);
 

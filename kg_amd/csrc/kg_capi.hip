@@ -65,6 +65,9 @@ struct kg_handle {
   std::vector<std::pair<int, size_t>> ev_marks;  // (kind, index of start event)
   double prof_ms[2] = {0.0, 0.0};
   int64_t prof_n[2] = {0, 0};
+  void* batch_dev = nullptr;  // descriptors of batched launches led by this handle
+  size_t batch_cap = 0;
+  hipEvent_t batch_done = nullptr;
 };
 
 namespace {
@@ -420,6 +423,161 @@ kg_status xfer(kg_handle* h, const State5& st, double* host, int32_t layout, int
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Batched RK4: every handle of a batch advances with one launch per stage.
+// ---------------------------------------------------------------------------
+StageArgs make_stage_args(kg_handle* h, int mode, const Box& b, const Roles& rl) {
+  StageArgs a{};
+  a.G = h->G;
+  a.tab = h->d_tab;
+  a.per = (int64_t)h->G.sys_stride * h->nsys;
+  for (int r = 0; r < R_NUM; ++r) {
+    const State5* st = rl.r[r];
+    a.rb[r] = st ? st->alloc : nullptr;
+    a.ri4[r] = st && st->f[4] ? (int)((st->f[4] - st->alloc) / a.per) : 4;
+  }
+  a.i_lo = b.i_lo;
+  a.i_hi = b.i_hi;
+  a.j_lo = b.j_lo;
+  a.j_hi = b.j_hi;
+  a.k_lo = b.k_lo;
+  a.k_hi = b.k_hi;
+  a.nsys = h->nsys;
+  return a;
+}
+
+kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
+  kg_handle* lead = hs[0];
+  const int ef = lead->G.metric_ef;
+  for (int q = 0; q < n; ++q) {
+    kg_handle* h = hs[q];
+    if (!h) return fail(lead, KG_ERR_INVALID, "batch: null handle %d", q);
+    if (h->desc.scheme != KG_RK4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d is not RK4", q);
+    if (h->G.metric_ef != ef) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed metric forms");
+    if (h->device != lead->device) return fail(lead, KG_ERR_INVALID, "batch: handles on different devices");
+    for (int p = 0; p < q; ++p)
+      if (hs[p] == h) return fail(lead, KG_ERR_INVALID, "batch: handle %d repeated", q);
+    KG_TRY(ensure_rk(h, true));
+    if (h != lead) KG_HIP(lead, hipStreamSynchronize(h->stream));
+  }
+  // descriptors for both buffer parities: version v has u and the alternate
+  // buffer of every handle exchanged v times
+  const int nst = 4;
+  const int modes[nst] = {M_RK4_S1, M_RK4_S2, M_RK4_S3, M_RK4_S4};
+  std::vector<StageArgs> sd(2 * nst * n);
+  std::vector<FillArgs> fd(2 * n);
+  std::vector<int> tstart(n + 1), bstart(3 * (n + 1));
+  std::vector<int64_t> nitems(3 * n);
+  for (int v = 0; v < 2; ++v)
+    for (int q = 0; q < n; ++q) {
+      kg_handle* h = hs[q];
+      const State5& P = v ? h->A : h->u;
+      const State5& X = v ? h->u : h->A;
+      const Box fb = filled_box(h->G);
+      Roles rl[nst] = {roles(&P, nullptr, &X), roles(&P, nullptr, &h->Ua, nullptr, &X),
+                       roles(&P, nullptr, &h->Ub, nullptr, &h->Ua), roles(&P, &P, &X, &X, &h->Ub, &h->Ua)};
+      for (int st = 0; st < nst; ++st) {
+        StageArgs a = make_stage_args(h, modes[st], fb, rl[st]);
+        a.kc = lead->tune_kc;
+        a.tj = lead->tune_tj;
+        finalize_tiles(a);
+        sd[(v * nst + st) * n + q] = a;
+      }
+      FillArgs f{};
+      f.G = h->G;
+      for (int c = 0; c < 5; ++c) {
+        f.u[c] = X.f[c];
+        f.u_p[c] = P.f[c];
+      }
+      fd[v * n + q] = f;
+    }
+  tstart[0] = 0;
+  for (int q = 0; q < n; ++q) {
+    const StageArgs& a = sd[q];
+    tstart[q + 1] = tstart[q] + a.nti * a.ntj * a.ntk * a.nsys;
+  }
+  for (int ph = 0; ph < 3; ++ph) {
+    bstart[ph * (n + 1)] = 0;
+    for (int q = 0; q < n; ++q) {
+      const int64_t it = fill_work_items(hs[q]->G, hs[q]->nsys, ph);
+      nitems[ph * n + q] = it;
+      bstart[ph * (n + 1) + q + 1] = bstart[ph * (n + 1) + q] + (int)((it + 255) / 256);
+    }
+  }
+  // one device block: [sd][fd][tstart][bstart][nitems]
+  const size_t o_fd = sd.size() * sizeof(StageArgs);
+  const size_t o_ts = o_fd + fd.size() * sizeof(FillArgs);
+  const size_t o_bs = o_ts + tstart.size() * sizeof(int);
+  const size_t o_ni = (o_bs + bstart.size() * sizeof(int) + 7) / 8 * 8;
+  const size_t bytes = o_ni + nitems.size() * sizeof(int64_t);
+  std::vector<char> host(bytes);
+  memcpy(host.data(), sd.data(), o_fd);
+  memcpy(host.data() + o_fd, fd.data(), fd.size() * sizeof(FillArgs));
+  memcpy(host.data() + o_ts, tstart.data(), tstart.size() * sizeof(int));
+  memcpy(host.data() + o_bs, bstart.data(), bstart.size() * sizeof(int));
+  memcpy(host.data() + o_ni, nitems.data(), nitems.size() * sizeof(int64_t));
+  if (lead->batch_cap < bytes) {
+    KG_HIP(lead, hipStreamSynchronize(lead->stream));
+    if (lead->batch_dev) KG_HIP(lead, hipFree(lead->batch_dev));
+    lead->batch_dev = nullptr;
+    lead->batch_cap = 0;
+    KG_HIP(lead, hipMalloc(&lead->batch_dev, bytes));
+    lead->batch_cap = bytes;
+  }
+  char* dev = (char*)lead->batch_dev;
+  KG_HIP(lead, hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, lead->stream));
+  const StageArgs* dsd = (const StageArgs*)dev;
+  const FillArgs* dfd = (const FillArgs*)(dev + o_fd);
+  const int* dts = (const int*)(dev + o_ts);
+  const int* dbs = (const int*)(dev + o_bs);
+  const int64_t* dni = (const int64_t*)(dev + o_ni);
+  for (int it = 0; it < nsteps; ++it) {
+    const int v = it & 1;
+    for (int st = 0; st < nst; ++st) {
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (lead->prof) {
+        e0 = next_event(lead);
+        e1 = next_event(lead);
+        if (!e0 || !e1) return fail(lead, KG_ERR_HIP, "hipEventCreate failed");
+        KG_HIP(lead, hipEventRecord(e0, lead->stream));
+      }
+      KG_HIP(lead, launch_stage_batch(modes[st], dsd + (size_t)(v * nst + st) * n, dts, n, tstart[n],
+                                      lead->tune_tj, ef, lead->stream));
+      if (lead->prof) {
+        KG_HIP(lead, hipEventRecord(e1, lead->stream));
+        lead->ev_marks.push_back({0, lead->ev_used - 2});
+      }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (lead->prof) {
+      e0 = next_event(lead);
+      e1 = next_event(lead);
+      if (!e0 || !e1) return fail(lead, KG_ERR_HIP, "hipEventCreate failed");
+      KG_HIP(lead, hipEventRecord(e0, lead->stream));
+    }
+    for (int ph = 0; ph < 3; ++ph)
+      KG_HIP(lead, launch_fill_batch(dfd + (size_t)v * n, dbs + ph * (n + 1), n, dni + ph * n,
+                                     bstart[ph * (n + 1) + n], ph, lead->stream));
+    if (lead->prof) {
+      KG_HIP(lead, hipEventRecord(e1, lead->stream));
+      lead->ev_marks.push_back({1, lead->ev_used - 2});
+    }
+  }
+  // bookkeeping and ordering: later work on any member waits for the batch
+  if (!lead->batch_done) KG_HIP(lead, hipEventCreateWithFlags(&lead->batch_done, hipEventDisableTiming));
+  KG_HIP(lead, hipEventRecord(lead->batch_done, lead->stream));
+  for (int q = 0; q < n; ++q) {
+    kg_handle* h = hs[q];
+    if (h != lead) KG_HIP(lead, hipStreamWaitEvent(h->stream, lead->batch_done, 0));
+    for (int it = 0; it < nsteps; ++it) h->t = h->t + h->desc.dt;
+    if (nsteps & 1) {
+      std::swap(h->u, h->A);
+      h->px ^= 1;
+    }
+  }
+  return KG_OK;
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -513,6 +671,8 @@ kg_status kg_destroy(kg_handle* h) {
   for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
+  if (h->batch_dev) (void)hipFree(h->batch_dev);
+  if (h->batch_done) (void)hipEventDestroy(h->batch_done);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return KG_OK;
@@ -604,6 +764,12 @@ kg_status kg_rk4(kg_handle* h) { return h ? do_steps(h, KG_RK4, 1) : fail(nullpt
 kg_status kg_step(kg_handle* h, int32_t nsteps) {
   if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
   return do_steps(h, h->desc.scheme, nsteps);
+}
+
+kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps) {
+  if (!hs || n < 1 || !hs[0]) return fail(nullptr, KG_ERR_INVALID, "batch: need at least one handle");
+  if (nsteps < 0) return fail(hs[0], KG_ERR_INVALID, "nsteps < 0");
+  return batch_step(hs, n, nsteps);
 }
 
 kg_status kg_sync(kg_handle* h) {

@@ -112,6 +112,16 @@ struct FillArgs {
 // launchers (kg_kernels.hip)
 hipError_t launch_stage(int mode, const StageArgs& a, hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase /*0 x,1 y,2 z*/, int nsys);
+// Batched launches: one launch covers every grid of descs[0..ngrids).
+// Stage: grid q owns workgroups [tstart[q], tstart[q+1]), its tiles times its
+// systems (finalize_tiles sets the tile counts).  RK4 stages only.
+void finalize_tiles(StageArgs& a);
+hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int tj,
+                              int ef, hipStream_t s);
+// Fill pass `phase`: grid q owns blocks [bstart[q], ...) of 256 work items, nitems[q] items.
+int64_t fill_work_items(const Geom& G, int nsys, int phase);
+hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrids, const int64_t* nitems,
+                             int nblocks, int phase, hipStream_t s);
 // dense [sys][5][nk][nj][ni] (device) <-> pitched fields; to_dev = 1: dense -> fields
 hipError_t launch_pack(const Geom& G, double* dense, double* const fields[5], int nsys, int to_dev, hipStream_t s);
 

@@ -79,7 +79,7 @@ struct Combine {
 // RK4 stage 4 holds the most operands; asking for 4 waves per SIMD (a few
 // spilled registers) measured faster than the 3 its natural allocation gives.
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kernel(const StageArgs a) {
+__device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, const int sysidx) {
   constexpr bool N14 = ModeTraits<MODE>::n14;
   constexpr bool N5 = ModeTraits<MODE>::n5;
   constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
@@ -95,8 +95,6 @@ __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kern
   __shared__ double sFT[TJ + 2][TI];  // theta-flux of rhs_pi
 
   const Geom& G = a.G;
-  const int nb = a.nti * a.ntj * a.ntk;
-  const int tile = remap_xcd((int)blockIdx.x, nb);
   const int ti = tile % a.nti;
   const int tj = (tile / a.nti) % a.ntj;
   const int tk = tile / (a.nti * a.ntj);
@@ -111,7 +109,7 @@ __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kern
   const bool act = valid && ii < a.i_hi && jj < a.j_hi;
   const int k0 = a.k_lo + tk * a.kc;
   const int k1 = min(k0 + a.kc, a.k_hi);
-  const int64_t sys = (int64_t)blockIdx.y * G.sys_stride;
+  const int64_t sys = (int64_t)sysidx * G.sys_stride;
   const uint32_t sysb = (uint32_t)(sys * 8);  // system offset in bytes
   const double* tab = a.tab;
   auto T = [&](int which, uint32_t off) { return gld(tab, off + (uint32_t)which * n2b); };
@@ -453,67 +451,122 @@ __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kern
   }
 }
 
+// One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.
+template <int MODE, int EF, int TJ>
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kernel(const StageArgs a) {
+  stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
+}
+
+// Many grids in one launch: grid q owns the workgroups [tstart[q], tstart[q+1])
+// (its tiles times its systems); the lookup is a uniform binary search.
+__device__ __forceinline__ int find_grid(const int* __restrict__ start, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int MODE, int EF, int TJ>
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kernel_batch(
+    const StageArgs* __restrict__ descs, const int* __restrict__ tstart, int ngrids, int total) {
+  const int b = remap_xcd((int)blockIdx.x, total);
+  const int q = find_grid(tstart, ngrids, b);
+  const StageArgs& a = descs[q];
+  const int local = b - tstart[q];
+  const int nt = a.nti * a.ntj * a.ntk;
+  stage_body<MODE, EF, TJ>(a, local % nt, local / nt);
+}
+
 // ---------------------------------------------------------------------------
 // Ghost fill (lines_method_lib.f90:313-330,507-597), three ordered passes.
 // Each pass only reads points it does not write.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void fill_x_kernel(const FillArgs a) {
+// Work items of each fill pass of one grid (all fields and systems).
+__host__ __device__ inline int64_t fill_items(const Geom& G, int nsys, int phase) {
+  if (phase == 0) return (int64_t)G.nj * G.nk * 5 * nsys;
+  if (phase == 1) return (int64_t)G.ni * 2 * (G.g + 1) * G.nk * 5 * nsys;
+  return G.plane * 2 * 5 * nsys;
+}
+
+template <int PHASE>
+__device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
   const Geom& G = a.G;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = blockIdx.z % 5;
-  const int64_t sys = (int64_t)(blockIdx.z / 5) * G.sys_stride;
-  if (t >= (int64_t)G.nj * G.nk) return;
-  const int jj = (int)(t % G.nj), kk = (int)(t / G.nj);
-  double* u = a.u[c] + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
-  // boundary_xL (:516-517)
-  u[0] = 3.0 * u[1] - 3.0 * u[2] + u[3];
-  // boundary_xR (:525-528)
-  if (G.bc_inflow) {
-    const double* up = a.u_p[c] + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
-    u[G.ni - 1] = up[G.ni - 1];
+  if constexpr (PHASE == 0) {
+    // boundary_xL (:516-517) and boundary_xR (:525-528)
+    const int jj = (int)(e % G.nj);
+    int64_t r = e / G.nj;
+    const int kk = (int)(r % G.nk);
+    r /= G.nk;
+    const int c = (int)(r % 5);
+    const int64_t sys = (r / 5) * G.sys_stride;
+    double* u = a.u[c] + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
+    u[0] = 3.0 * u[1] - 3.0 * u[2] + u[3];
+    if (G.bc_inflow) {
+      const double* up = a.u_p[c] + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
+      u[G.ni - 1] = up[G.ni - 1];
+    } else {
+      u[G.ni - 1] = u[G.ni - 2];
+    }
+  } else if constexpr (PHASE == 1) {
+    // boundary_yL (:542-555) and boundary_yR (:565-578): whole r-lines,
+    // rows -gg <- gg+1 and Ny+gg <- Ny-gg-1, taken at k_target = mod(k+Nz/2, Nz-1)
+    const int nrow = G.g + 1;
+    const int ii = (int)(e % G.ni);
+    int64_t r = e / G.ni;
+    const int row = (int)(r % (2 * nrow));
+    r /= 2 * nrow;
+    const int kk = (int)(r % G.nk);
+    r /= G.nk;
+    const int c = (int)(r % 5);
+    const int64_t sys = (r / 5) * G.sys_stride;
+    const int kt = (kk + G.Nz / 2) % (G.Nz - 1);
+    int jd, js;
+    if (row < nrow) {
+      jd = G.g - row;          // j = -gg
+      js = G.g + row + 1;      // j = gg+1
+    } else {
+      const int gg = row - nrow;
+      jd = G.g + G.Ny + gg;       // j = Ny+gg
+      js = G.g + G.Ny - gg - 1;   // j = Ny-gg-1
+    }
+    double* u = a.u[c] + sys;
+    u[(int64_t)kk * G.plane + (int64_t)jd * G.ld + ii] = u[(int64_t)kt * G.plane + (int64_t)js * G.ld + ii];
   } else {
-    u[G.ni - 1] = u[G.ni - 2];
+    // boundary_zL (:587) and boundary_zR (:596)
+    const int64_t t = e % G.plane;
+    int64_t r = e / G.plane;
+    const int side = (int)(r % 2);
+    r /= 2;
+    const int c = (int)(r % 5);
+    const int64_t sys = (r / 5) * G.sys_stride;
+    double* u = a.u[c] + sys;
+    if (side == 0)
+      u[t] = u[(int64_t)(G.Nz - 1) * G.plane + t];
+    else
+      u[(int64_t)G.Nz * G.plane + t] = u[G.plane + t];
   }
 }
 
-__global__ __launch_bounds__(256) void fill_y_kernel(const FillArgs a) {
-  // boundary_yL (:542-555) and boundary_yR (:565-578): whole r-lines,
-  // rows -gg <- gg+1 and Ny+gg <- Ny-gg-1, taken at k_target = mod(k+Nz/2, Nz-1)
-  const Geom& G = a.G;
-  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ii >= G.ni) return;
-  const int nrow = G.g + 1;
-  const int row = blockIdx.y % (2 * nrow);
-  const int kk = blockIdx.y / (2 * nrow);
-  const int c = blockIdx.z % 5;
-  const int64_t sys = (int64_t)(blockIdx.z / 5) * G.sys_stride;
-  const int kt = (kk + G.Nz / 2) % (G.Nz - 1);
-  int jd, js;
-  if (row < nrow) {
-    const int gg = row;
-    jd = G.g - gg;          // j = -gg
-    js = G.g + gg + 1;      // j = gg+1
-  } else {
-    const int gg = row - nrow;
-    jd = G.g + G.Ny + gg;       // j = Ny+gg
-    js = G.g + G.Ny - gg - 1;   // j = Ny-gg-1
-  }
-  double* u = a.u[c] + sys;
-  u[(int64_t)kk * G.plane + (int64_t)jd * G.ld + ii] = u[(int64_t)kt * G.plane + (int64_t)js * G.ld + ii];
+// The three passes run in order (xL/xR, yL/yR, zL/zR; lines_method_lib.f90:313-330);
+// each reads only points it does not write.
+template <int PHASE>
+__global__ __launch_bounds__(256) void fill_kernel(const FillArgs a, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) fill_item<PHASE>(a, e);
 }
 
-__global__ __launch_bounds__(256) void fill_z_kernel(const FillArgs a) {
-  // boundary_zL (:587) and boundary_zR (:596)
-  const Geom& G = a.G;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= G.plane) return;
-  const int c = blockIdx.z % 5;
-  const int64_t sys = (int64_t)(blockIdx.z / 5) * G.sys_stride;
-  double* u = a.u[c] + sys;
-  if (blockIdx.y == 0)
-    u[t] = u[(int64_t)(G.Nz - 1) * G.plane + t];
-  else
-    u[(int64_t)G.Nz * G.plane + t] = u[G.plane + t];
+template <int PHASE>
+__global__ __launch_bounds__(256) void fill_kernel_batch(const FillArgs* __restrict__ descs,
+                                                         const int* __restrict__ bstart, int ngrids,
+                                                         const int64_t* __restrict__ nitems) {
+  const int q = find_grid(bstart, ngrids, (int)blockIdx.x);
+  const int64_t e = (int64_t)(blockIdx.x - bstart[q]) * 256 + threadIdx.x;
+  if (e < nitems[q]) fill_item<PHASE>(descs[q], e);
 }
 
 // dense [sys][5][nk][nj][ni] <-> pitched fields
@@ -551,16 +604,47 @@ hipError_t launch_mode(const StageArgs& a, hipStream_t s, int nsys) {
   return a.tj == 8 ? launch_mode_tj<MODE, 8>(a, s, nsys) : launch_mode_tj<MODE, 4>(a, s, nsys);
 }
 
+template <int MODE, int TJ>
+hipError_t launch_batch_tj(const StageArgs* descs, const int* tstart, int ngrids, int total, int ef, hipStream_t s) {
+  dim3 grid((unsigned)total, 1, 1);
+  dim3 block(TI, TJ, 1);
+  if (ef)
+    hipLaunchKernelGGL((stage_kernel_batch<MODE, 1, TJ>), grid, block, 0, s, descs, tstart, ngrids, total);
+  else
+    hipLaunchKernelGGL((stage_kernel_batch<MODE, 0, TJ>), grid, block, 0, s, descs, tstart, ngrids, total);
+  return hipGetLastError();
+}
+
 }  // namespace
 
-hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
-  StageArgs a = a0;
-  const int nsys = a.nsys;
+void finalize_tiles(StageArgs& a) {
   if (a.tj != 4 && a.tj != 8) a.tj = 4;
   if (a.kc < 1) a.kc = 1;
   a.nti = (a.i_hi - a.i_lo + TI - 1) / TI;
   a.ntj = (a.j_hi - a.j_lo + a.tj - 1) / a.tj;
   a.ntk = (a.k_hi - a.k_lo + a.kc - 1) / a.kc;
+}
+
+hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int tj,
+                              int ef, hipStream_t s) {
+  const bool t8 = tj == 8;
+  switch (mode) {
+    case M_RK4_S1: return t8 ? launch_batch_tj<M_RK4_S1, 8>(descs, tstart, ngrids, total, ef, s)
+                             : launch_batch_tj<M_RK4_S1, 4>(descs, tstart, ngrids, total, ef, s);
+    case M_RK4_S2: return t8 ? launch_batch_tj<M_RK4_S2, 8>(descs, tstart, ngrids, total, ef, s)
+                             : launch_batch_tj<M_RK4_S2, 4>(descs, tstart, ngrids, total, ef, s);
+    case M_RK4_S3: return t8 ? launch_batch_tj<M_RK4_S3, 8>(descs, tstart, ngrids, total, ef, s)
+                             : launch_batch_tj<M_RK4_S3, 4>(descs, tstart, ngrids, total, ef, s);
+    case M_RK4_S4: return t8 ? launch_batch_tj<M_RK4_S4, 8>(descs, tstart, ngrids, total, ef, s)
+                             : launch_batch_tj<M_RK4_S4, 4>(descs, tstart, ngrids, total, ef, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
+  StageArgs a = a0;
+  const int nsys = a.nsys;
+  finalize_tiles(a);
   switch (mode) {
     case M_RK4_S1: return launch_mode<M_RK4_S1>(a, s, nsys);
     case M_RK4_S2: return launch_mode<M_RK4_S2>(a, s, nsys);
@@ -580,18 +664,30 @@ hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
 }
 
 hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase, int nsys) {
-  const Geom& G = a.G;
-  if (phase == 0) {
-    const int64_t n = (int64_t)G.nj * G.nk;
-    hipLaunchKernelGGL(fill_x_kernel, dim3((unsigned)((n + 255) / 256), 1, 5 * nsys), dim3(256), 0, s, a);
-  } else if (phase == 1) {
-    hipLaunchKernelGGL(fill_y_kernel, dim3((unsigned)((G.ni + 63) / 64), 2 * (G.g + 1) * G.nk, 5 * nsys),
-                       dim3(64), 0, s, a);
-  } else {
-    hipLaunchKernelGGL(fill_z_kernel, dim3((unsigned)((G.plane + 255) / 256), 2, 5 * nsys), dim3(256), 0, s, a);
-  }
+  const int64_t n = fill_items(a.G, nsys, phase);
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (phase == 0)
+    hipLaunchKernelGGL(fill_kernel<0>, grid, dim3(256), 0, s, a, n);
+  else if (phase == 1)
+    hipLaunchKernelGGL(fill_kernel<1>, grid, dim3(256), 0, s, a, n);
+  else
+    hipLaunchKernelGGL(fill_kernel<2>, grid, dim3(256), 0, s, a, n);
   return hipGetLastError();
 }
+
+hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrids, const int64_t* nitems,
+                             int nblocks, int phase, hipStream_t s) {
+  const dim3 grid((unsigned)nblocks);
+  if (phase == 0)
+    hipLaunchKernelGGL(fill_kernel_batch<0>, grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
+  else if (phase == 1)
+    hipLaunchKernelGGL(fill_kernel_batch<1>, grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
+  else
+    hipLaunchKernelGGL(fill_kernel_batch<2>, grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
+  return hipGetLastError();
+}
+
+int64_t fill_work_items(const Geom& G, int nsys, int phase) { return fill_items(G, nsys, phase); }
 
 hipError_t launch_pack(const Geom& G, double* src, double* const d[5], int nsys, int to_dev, hipStream_t s) {
   hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((G.ni + 63) / 64 * G.nj), (unsigned)G.nk, 5 * nsys), dim3(64),

@@ -98,3 +98,35 @@ def random_lm_coeffs(seed, lmax=4, std=1e-3):
 def config2_state(seed=20261203, Nx=2048, Ny=129, Nz=257, g=2, xmin=1.8, xmax=200.0):
     """phi = exp(-(r-30)^2/3^2) sum_{l<=4} c_lm Y_lm, c_lm ~ N(0, 1e-3) (the full single-GPU workload)."""
     return shell_state(Nx, Ny, Nz, g, xmin, xmax, 30.0, 3.0, 1.0, random_lm_coeffs(seed))
+
+
+def shell_state_device(Nx, Ny, Nz, g, xmin, xmax, r0, width, amp, coeffs, device="cuda"):
+    """shell_state built on the GPU with torch (same values, SoA [5][nk][nj][ni]).
+
+    The radial profile and the angular sum are formed on the host in FP64 (small
+    1-D/2-D arrays) and their outer products on the device, so large inputs do
+    not pass through host memory.
+    """
+    import torch
+
+    x, y, z = grid_coords(Nx, Ny, Nz, g, xmin, xmax)
+    R = amp * np.exp(-((x - r0) ** 2) / width ** 2)
+    dR = R * (-2.0 * (x - r0) / width ** 2)
+    A = np.zeros((len(y), len(z)))
+    At = np.zeros_like(A)
+    Ap = np.zeros_like(A)
+    for (l, m), c in coeffs.items():
+        Y, Yt, Yp = real_ylm(l, m, y, z)
+        A += c * Y
+        At += c * Yt
+        Ap += c * Yp
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=torch.float64)
+    Rt, dRt = t(R), t(dR)
+    At_, Att, Apt = t(A.T), t(At.T), t(Ap.T)  # [nk][nj]
+    u = torch.empty((5, len(z), len(y), len(x)), dtype=torch.float64, device=device)
+    u[0] = At_[:, :, None] * Rt[None, None, :]
+    u[1] = At_[:, :, None] * dRt[None, None, :]
+    u[2] = Att[:, :, None] * Rt[None, None, :]
+    u[3] = Apt[:, :, None] * Rt[None, None, :]
+    u[4] = 0.0
+    return u
