@@ -252,3 +252,89 @@ class GenericIntegrator:
         fn = getattr(lib(), "kgo_%s_generic" % scheme)
         for _ in range(nsteps):
             fn(C.byref(self.ops), C.byref(self.b), dt)
+
+
+# --------------------------------------------------------------------------
+# Spectral projection (kg_spectral_oracle.c)
+# --------------------------------------------------------------------------
+def _spec():
+    L = lib()
+    if not getattr(L, "_spec_ready", False):
+        L.kgs_plm.restype = C.c_double
+        L.kgs_plm.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.kgs_sph_jl.restype = C.c_double
+        L.kgs_sph_jl.argtypes = [C.c_int, C.c_double]
+        L.kgs_jl_zero.restype = C.c_double
+        L.kgs_jl_zero.argtypes = [C.c_int, C.c_int]
+        L.kgs_sph_harmonics.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.kgs_project_sh.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                                     _dp, _dp]
+        L.kgs_bessel_basis.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double, C.c_int, C.c_double, _dp, _dp]
+        L.kgs_radial_project.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.kgs_domain_to_sph.argtypes = [C.c_int] * 4 + [C.c_double] * 4 + [_dp, _dp]
+        L._spec_ready = True
+    return L
+
+
+def plm(l, m, x):
+    return _spec().kgs_plm(l, m, x)
+
+
+def sph_jl(l, x):
+    return _spec().kgs_sph_jl(l, x)
+
+
+def jl_zero(l, n):
+    return _spec().kgs_jl_zero(l, n)
+
+
+def sph_harmonics(lmax, theta, phi):
+    """Y_lm on the spectral grid: complex [lm][len(theta)][len(phi)], lm = l*l + m + l."""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    n = (lmax + 1) ** 2
+    yr = np.zeros((n, len(theta), len(phi)))
+    yi = np.zeros_like(yr)
+    _spec().kgs_sph_harmonics(lmax, len(theta), len(phi), _ptr(theta), _ptr(phi), _ptr(yr), _ptr(yi))
+    return yr + 1j * yi
+
+
+def project_sh(f, ylm, lmax, theta, dth, dph):
+    """project_sh of one complex slice f[theta][phi] -> flm[lm] complex."""
+    f = np.asarray(f, dtype=np.complex128)
+    fr, fi = np.ascontiguousarray(f.real), np.ascontiguousarray(f.imag)
+    yr, yi = np.ascontiguousarray(ylm.real), np.ascontiguousarray(ylm.imag)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    n = (lmax + 1) ** 2
+    orr, oi = np.zeros(n), np.zeros(n)
+    _spec().kgs_project_sh(f.shape[0], f.shape[1], dth, dph, _ptr(theta), _ptr(fr), _ptr(fi), lmax, _ptr(yr),
+                           _ptr(yi), _ptr(orr), _ptr(oi))
+    return orr + 1j * oi
+
+
+def bessel_basis(lmax, nk, r, R_max, kind=0, kmax=0.0):
+    """(k_values [l][n], j_basis [l][n][i])."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    kv = np.zeros((lmax + 1, nk))
+    jb = np.zeros((lmax + 1, nk, len(r)))
+    _spec().kgs_bessel_basis(lmax, nk, len(r), _ptr(r), R_max, kind, kmax, _ptr(kv), _ptr(jb))
+    return kv, jb
+
+
+def radial_project(lmax, nk, r, dr, jb, flm):
+    """flm [i][lm] complex -> ft [n][lm] complex."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    fr, fi = np.ascontiguousarray(flm.real), np.ascontiguousarray(flm.imag)
+    jb = np.ascontiguousarray(jb, dtype=np.float64)
+    n = (lmax + 1) ** 2
+    orr, oi = np.zeros((nk, n)), np.zeros((nk, n))
+    _spec().kgs_radial_project(lmax, nk, len(r), _ptr(r), dr, _ptr(jb), _ptr(fr), _ptr(fi), _ptr(orr), _ptr(oi))
+    return orr + 1j * oi
+
+
+def domain_to_sph(fd, Nx, Ny, Nz, g, xmin, dx, dy, dz):
+    """Padded real field fd[nk][nj][ni] -> spectral grid f[Nz][Ny][Nx]."""
+    fd = np.ascontiguousarray(fd, dtype=np.float64)
+    out = np.zeros((Nz, Ny, Nx))
+    _spec().kgs_domain_to_sph(Nx, Ny, Nz, g, xmin, dx, dy, dz, _ptr(fd), _ptr(out))
+    return out
