@@ -148,6 +148,60 @@ kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj);
 kg_status kg_profile_enable(kg_handle* h, int32_t on);
 kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t* launches);
 
+/* ------------------------------------------------------------------------
+ * Spherical-Bessel spectral projection (tools/FFT_sph.f90), on the grid of a
+ * handle.  The spectral grid is the reference's node-centred one
+ * (mesh_refinement_lib.f90:135-143): r_i = xmin + i dx (i < Nx),
+ * theta_j = j dtheta (j < Ny), phi_k = k dphi (k < Nz).  Indices lm =
+ * l*l + l + m for l = 0..lmax, m = -l..l.  Complex outputs are interleaved
+ * (re, im) doubles.  Outputs with on_device = 1 are device pointers filled
+ * asynchronously on the handle's stream; otherwise host arrays, filled
+ * before the call returns.
+ * ------------------------------------------------------------------------ */
+typedef struct kg_spec kg_spec;
+/* radial basis: KG_BASIS_ZEROS = generate_full_bessel_basis (FFT_sph.f90:13-95):
+ * k_{l,n} = z_n(J_{l+1/2}) / R_max, j_l(k r) normalised to unit trapezoid
+ * norm with weight r^2;  KG_BASIS_UNIFORM: k_n = kmax n / nk for every l,
+ * j_l(k r) unnormalised. */
+enum { KG_BASIS_ZEROS = 0, KG_BASIS_UNIFORM = 1 };
+/* arithmetic of the radial contraction: FP64, or BF16 matrix cores with
+ * split operands (five 8-bit slices per operand, exact FP32 accumulation of
+ * each slice product over 32-term chunks, chunks combined in FP64) */
+enum { KG_SPEC_FP64 = 0, KG_SPEC_SPLIT_BF16 = 1 };
+typedef struct kg_spec_desc {
+  int32_t lmax;  /* l_max (1..16) */
+  int32_t nk;    /* Nk: radial wave numbers n = 1..nk */
+  int32_t basis; /* KG_BASIS_* */
+  int32_t reserved;
+  double R_max;  /* KG_BASIS_ZEROS: normalisation radius (the reference's r_max) */
+  double kmax;   /* KG_BASIS_UNIFORM: largest wave number */
+} kg_spec_desc;
+
+/* scalar_field_constructor's spectral set-up (diagnostic_lib.f90:144-149):
+ * Y_lm tables (sph_harmonics, FFT_sph.f90:282-336) and the Bessel basis, on
+ * h's device.  The projector reads h's current state at each call. */
+kg_status kg_spec_create(kg_handle* h, const kg_spec_desc* desc, kg_spec** out);
+kg_status kg_spec_destroy(kg_spec* p);
+const char* kg_spec_last_error(const kg_spec* p);
+/* Host copies of the set-up tables; any pointer may be NULL.
+ * ylm: complex [nlm][Ny][Nz] (sph_harmonics' Ylm(l,m,j,k));  kv: [lmax+1][nk]
+ * (k_values);  jb: [lmax+1][nk][Nx] (j_basis). */
+kg_status kg_spec_get_tables(kg_spec* p, double* ylm, double* kv, double* jb);
+/* domain_to_sph (FFT_sph.f90:525-651) of state field `field` (0..4):
+ * out = real [nsys][Nz][Ny][Nx] (nsys = 2 for a complex state: real part,
+ * then imaginary part). */
+kg_status kg_spec_domain_to_sph(kg_spec* p, int32_t field, double* out, int32_t on_device);
+/* project_sh (FFT_sph.f90:247-279) at every spectral radius, as
+ * SphFT_discrete_radial applies it (:228-231): flm = complex [nlm][Nx]. */
+kg_status kg_spec_project_sh(kg_spec* p, int32_t field, double* flm, int32_t on_device);
+/* The radial contraction SphFT_discrete_radial defines (:233-236):
+ * ft(n,l,m) = trapezoid_i( flm(r_i) j_l(k_n r_i) r_i^2, dr ), computed from
+ * domain_to_sph + project_sh of `field`: ft = complex [nlm][nk]. */
+kg_status kg_spec_radial(kg_spec* p, int32_t field, int32_t precision, double* ft, int32_t on_device);
+/* The radial contraction alone, on caller-supplied host flm (complex
+ * [nlm][Nx]) -> host ft (complex [nlm][nk]). */
+kg_status kg_spec_radial_flm(kg_spec* p, const double* flm, int32_t precision, double* ft);
+
 #ifdef __cplusplus
 }
 #endif

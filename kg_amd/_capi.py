@@ -33,6 +33,18 @@ class KGDesc(C.Structure):
     ]
 
 
+KG_BASIS_ZEROS, KG_BASIS_UNIFORM = range(2)
+KG_SPEC_FP64, KG_SPEC_SPLIT_BF16 = range(2)
+
+
+class KGSpecDesc(C.Structure):
+    """Mirror of struct kg_spec_desc."""
+    _fields_ = [
+        ("lmax", C.c_int32), ("nk", C.c_int32), ("basis", C.c_int32), ("reserved", C.c_int32),
+        ("R_max", C.c_double), ("kmax", C.c_double),
+    ]
+
+
 SIGNATURES = {
     "kg_create": (C.c_int32, [C.POINTER(KGDesc), C.POINTER(_vp)]),
     "kg_destroy": (C.c_int32, [_vp]),
@@ -56,6 +68,14 @@ SIGNATURES = {
     "kg_set_tuning": (C.c_int32, [_vp, C.c_int32, C.c_int32]),
     "kg_profile_enable": (C.c_int32, [_vp, C.c_int32]),
     "kg_profile_read": (C.c_int32, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "kg_spec_create": (C.c_int32, [_vp, C.POINTER(KGSpecDesc), C.POINTER(_vp)]),
+    "kg_spec_destroy": (C.c_int32, [_vp]),
+    "kg_spec_last_error": (C.c_char_p, [_vp]),
+    "kg_spec_get_tables": (C.c_int32, [_vp, _vp, _vp, _vp]),
+    "kg_spec_domain_to_sph": (C.c_int32, [_vp, C.c_int32, _vp, C.c_int32]),
+    "kg_spec_project_sh": (C.c_int32, [_vp, C.c_int32, _vp, C.c_int32]),
+    "kg_spec_radial": (C.c_int32, [_vp, C.c_int32, C.c_int32, _vp, C.c_int32]),
+    "kg_spec_radial_flm": (C.c_int32, [_vp, _vp, C.c_int32, _vp]),
 }
 
 

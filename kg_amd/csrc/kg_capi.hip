@@ -818,3 +818,16 @@ kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t*
 }
 
 }  // extern "C"
+
+namespace kg {
+int handle_view(const kg_handle* h, HandleView* v) {
+  if (!h || !v) return 1;
+  v->device = h->device;
+  v->stream = h->stream;
+  v->G = h->G;
+  v->hg = h->hg;
+  v->nsys = h->nsys;
+  for (int f = 0; f < kNumFields; ++f) v->u[f] = h->u.f[f];
+  return 0;
+}
+}  // namespace kg

@@ -12,6 +12,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
+struct kg_handle;
+
 namespace kg {
 
 constexpr int kNumFields = 5;  // phi, psi_r, psi_theta, psi_phi, Pi
@@ -124,5 +128,22 @@ hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrid
                              int nblocks, int phase, hipStream_t s);
 // dense [sys][5][nk][nj][ni] (device) <-> pitched fields; to_dev = 1: dense -> fields
 hipError_t launch_pack(const Geom& G, double* dense, double* const fields[5], int nsys, int to_dev, hipStream_t s);
+
+// What the spectral projector needs from a handle: its device, stream, grid
+// and the current state's field buffers (these move between steps for RK4).
+struct HandleView {
+  int device;
+  hipStream_t stream;
+  Geom G;
+  HostGrid hg;
+  int nsys;
+  double* u[5];
+};
+int handle_view(const ::kg_handle* h, HandleView* v);
+
+// Special functions for the spectral tables (kg_special.cpp, host).
+double ylm_norm_plm(int l, int m, double x);  // N_lm P_l^m(x), Condon-Shortley phase
+double sph_jl(int l, double x);
+std::vector<double> jl_zeros(int lmax, int nk);  // [l][n]: the first nk positive zeros of j_l
 
 }  // namespace kg

@@ -451,9 +451,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
   }
 }
 
-// One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.
+// One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The RK4
+// final stage asks for 16/TJ resident workgroups per CU: 4 waves per SIMD, so
+// its 20 input streams keep up to 128 VGPRs and never spill to scratch.
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : 1)) void stage_kernel(const StageArgs a) {
   stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
 }
 
@@ -472,7 +474,7 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ start, int n, i
 }
 
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 4 : 1)) void stage_kernel_batch(
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : 1)) void stage_kernel_batch(
     const StageArgs* __restrict__ descs, const int* __restrict__ tstart, int ngrids, int total) {
   const int b = remap_xcd((int)blockIdx.x, total);
   const int q = find_grid(tstart, ngrids, b);

@@ -10,7 +10,8 @@
  *   P_l^m(x)  -- upward recurrence in l from P_m^m = (-1)^m (2m-1)!! (1-x^2)^(m/2)
  *                (Condon-Shortley phase included, as GSL's Plm);
  *   j_l(x)    -- upward recurrence from j_0, j_1 for x > l, otherwise Miller's
- *                downward recurrence normalised by j_0 (series for tiny x);
+ *                downward recurrence normalised by the larger of j_0, j_1
+ *                (series for tiny x);
  *   zeros of J_{l+1/2} = zeros of j_l -- bracketed on a grid and bisected to
  *                full precision.
  * Parity of these three is pinned against closed forms and scipy in tests/.
@@ -85,6 +86,12 @@ double kgs_sph_jl(int l, double x) {
       jp *= 1e-250;
       out *= 1e-250;
     }
+  }
+  /* normalise by whichever of j_0, j_1 is larger: near a zero of sin x the
+   * ratio j_0/jc alone loses every digit */
+  if (ax >= 1.0) {
+    const double j1 = sin(x) / (x * x) - cos(x) / x;
+    if (fabs(j1) > fabs(j0)) return out * (j1 / jp);
   }
   return out * (j0 / jc);
 }

@@ -64,6 +64,16 @@ def test_sph_jl_matches_scipy():
         assert err.max() < 1e-11, (l, xs[err.argmax()], err.max())
 
 
+def test_sph_jl_downward_recurrence_near_zeros_of_j0():
+    """x = n*pi below the order: Miller's recurrence normalised by j_0 alone would
+    divide by sin(x) ~ 1e-16 there."""
+    for l in (4, 7, 10):
+        for x in (math.pi, 2 * math.pi, 3 * math.pi, 3 * math.pi + 1e-9):
+            if x < l:
+                ref = spherical_jn(l, x)
+                assert oracle.sph_jl(l, x) == pytest.approx(ref, rel=1e-12), (l, x)
+
+
 def test_jl_zeros():
     for n in range(1, 8):
         assert oracle.jl_zero(0, n) == pytest.approx(n * math.pi, rel=1e-15)
