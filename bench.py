@@ -12,7 +12,13 @@ in HBM).  The timed steps run with a HIP event pair around every kernel on
 the engine's stream, which gives the average duration of the RHS+stage
 kernel for the roofline line.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Beside it, on the same GPU: the uneven batched workload (256 evolutions, one
+launch per RK4 stage, vs the same grids one at a time) with its
+spherical-Bessel projection (l <= 8, 512 k, split BF16 and FP64), the serving
+loop (Poisson requests, continuous batching, p50/p99 time-to-result), and
+the CPU oracle on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batched N] [--serve N]
 """
 from __future__ import annotations
 
@@ -57,6 +63,20 @@ def cpu_baseline(max_seconds=20.0):
                       "8 phi planes), 1 thread, %.1f s" % (steps, Nx, Ny - 1, Nz - 1, el)}
 
 
+def measured_traffic(tuning):
+    """HBM bytes per stage launch from the committed PMC profile of this workload
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a copy
+    of known size); None when the launch shape differs from the profiled one."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_rk4_nr2048_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        prof = json.load(f)
+    if (prof["tuning"]["kc"], prof["tuning"]["tj"]) != tuple(tuning):
+        return None
+    return prof["traffic_bytes_per_launch"]
+
+
 def batched_vs_plain(n, steps, tuning):
     """The uneven batched workload: n evolutions (Nr in [256, 2048], (Ntheta, Nphi)
     in {(16,32), (32,64), (64,128)}, m ~ U[0,0.5], M ~ U[0.5,2], RK4), advanced
@@ -89,13 +109,76 @@ def batched_vs_plain(n, steps, tuning):
         m.step(steps)
         m.sync()
         tp += time.perf_counter() - t0
+    spec_out = spectral_projection(members, tb / steps)
     for m in members:
         m.close()
     vb, vp = pts * steps / tb, pts * steps / tp
     return {"members": n, "steps": steps, "grid_points": pts, "batched_updates_per_s": vb,
             "plain_updates_per_s": vp, "speedup": vb / vp,
             "workload": "uneven RK4 batch: Nr~U{256..2048}, (Ntheta,Nphi) in {(16,32),(32,64),(64,128)}, "
-                        "m~U[0,0.5], M~U[0.5,2], seed 20261203"}
+                        "m~U[0,0.5], M~U[0.5,2], seed 20261203",
+            "spectral": spec_out}
+
+
+def spectral_projection(members, batched_step_s):
+    """The every-50-steps projection of the batched workload: phi of every member
+    onto j_l(k r), l <= 8, k_n = 4n/512 (n = 1..512), split-BF16 matrix cores
+    and FP64, all members back to back on the GPU (results left in HBM)."""
+    import numpy as np
+    import torch
+    from kg_amd.spectral import SpectralProjector
+
+    lmax, nk = 8, 512
+    projs = [SpectralProjector(m, lmax=lmax, nk=nk, basis="uniform", kmax=4.0) for m in members]
+    nlm = (lmax + 1) ** 2
+    outs = {p: torch.empty((len(projs), nlm * nk * 2), dtype=torch.float64, device="cuda")
+            for p in ("split_bf16", "fp64")}
+    times = {}
+    for prec in ("split_bf16", "fp64", "split_bf16", "fp64"):  # the first pass of each warms up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q, sp in enumerate(projs):
+            sp.radial(0, prec, out_device_ptr=outs[prec][q].data_ptr())
+        for m in members:
+            m.sync()
+        times[prec] = time.perf_counter() - t0
+    a, b = outs["split_bf16"].cpu().numpy(), outs["fp64"].cpu().numpy()
+    err = max(float(np.abs(a[q] - b[q]).max() / max(np.abs(b[q]).max(), 1e-300)) for q in range(len(projs)))
+    for sp in projs:
+        sp.close()
+    return {"lmax": lmax, "nk": nk, "members": len(projs),
+            "ms_split_bf16": times["split_bf16"] * 1e3, "ms_fp64": times["fp64"] * 1e3,
+            "max_rel_err_split_bf16_vs_fp64": err,
+            "fraction_of_50_steps": times["split_bf16"] / (50.0 * batched_step_s)}
+
+
+def serving(nreq, tuning):
+    """Serving loop: nreq requests of the batched workload arriving as a Poisson
+    stream (50/s), steps ~U{500..5000}, admitted against the free HBM and run by
+    continuous batching; each returns its final fields and its spectrum.  The
+    same requests one at a time through the plain path give the comparison."""
+    import torch
+    from kg_amd.serving import GpuEngine, ServingLoop, percentile, poisson_requests
+
+    reqs = poisson_requests(nreq)
+    eng = GpuEngine(tuning=tuning)
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(0.85 * free)
+    results, stats = ServingLoop(eng, budget, chunk=50).run(reqs, keep_fields=True)
+    lat = [r.latency for r in results]
+    del results
+    t0 = time.perf_counter()
+    for r in reqs:
+        eng.run_alone(r, keep_fields=True)
+    tp = time.perf_counter() - t0
+    ups_plain = stats.updates / tp
+    return {"requests": nreq, "arrival_rate_per_s": 50.0, "hbm_budget_bytes": budget,
+            "p50_time_to_result_s": percentile(lat, 50), "p99_time_to_result_s": percentile(lat, 99),
+            "served_updates_per_s": stats.updates_per_s, "one_at_a_time_updates_per_s": ups_plain,
+            "speedup": stats.updates_per_s / ups_plain, "max_active": stats.max_active,
+            "chunks": stats.chunks, "joined_running": stats.joined_running,
+            "workload": "Poisson(50/s) requests drawn as the batched workload, steps ~U{500..5000}, "
+                        "chunks of 50 RK4 steps, final fields + l<=8, 512-k spectrum (split-BF16)"}
 
 
 def main():
@@ -109,6 +192,8 @@ def main():
     ap.add_argument("--batched", type=int, default=256,
                     help="members of the uneven batched workload to time beside the plain path (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--serve", type=int, default=8,
+                    help="requests of the serving loop to run beside the plain path (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (a single grid lives on one MI355X); "
@@ -185,7 +270,7 @@ def main():
                    "grid_points": npts, "scheme": "rk4", "fields": 5, "state_finite": finite,
                    "tuning": {"kc": best[0], "tj": best[1]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(best),
                      "kernel": "stage_kernel (RHS + RK4 stage combine)",
                      "avg_kernel_ms": avg_stage_s * 1e3, "launches": stage_n,
                      "bytes_per_launch": per_launch_bytes},
@@ -194,6 +279,9 @@ def main():
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
         out["batched"] = batched_vs_plain(args.batched, args.batched_steps, best)
+    if args.serve > 0:
+        print("bench: serving loop, %d requests" % args.serve, file=sys.stderr, flush=True)
+        out["serving"] = serving(args.serve, best)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

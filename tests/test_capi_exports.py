@@ -41,6 +41,13 @@ def test_desc_struct_matches_header_layout():
     assert _capi.KGDesc.xmin.offset == 40
 
 
+def test_spec_desc_struct_matches_header_layout():
+    from kg_amd import _capi
+    # 4 int32 + 2 doubles
+    assert C.sizeof(_capi.KGSpecDesc) == 4 * 4 + 2 * 8
+    assert _capi.KGSpecDesc.R_max.offset == 16
+
+
 def test_create_without_gpu_fails_loudly():
     from conftest import gpu_available
     if gpu_available():
