@@ -1,13 +1,19 @@
 // kg_kernels.hip -- gfx950 kernels of the Klein-Gordon method-of-lines step.
 //
 // One launch per integrator stage: every thread owns one (r, theta) column
-// point (ii, jj) of a 64 x 4 tile and marches over a chunk of phi planes.  At
+// point (ii, jj) of a 64 x TJ tile and marches over a chunk of phi planes.  At
 // each point it evaluates the right-hand side of the five fields
 // (MoL/lines_method_lib.f90:671-753) with the reference's 2nd-order stencils
 // and edge rows (tools/finite_differences.f90:17-44,193-239,310-338,767-801)
 // and applies the stage combine of the integrator in the same expression.
 // The metric tables are k-independent, so a thread loads its coefficients
 // once for the whole phi march.
+//
+// Per plane: products into LDS, loads of the next plane issued, barrier,
+// derivatives from LDS, then the terms that need the next plane.  The loads
+// are kept as raw operands (u_p and k of an RK4 stage state) and combined
+// only after the barrier and the LDS work, so their latency overlaps both;
+// scheduling barriers keep the compiler from moving that arithmetic ahead.
 //
 // Floating-point contraction is off for this file (build flag
 // -ffp-contract=off): every product and sum is rounded on its own, in the
@@ -76,8 +82,6 @@ struct Combine {
   static constexpr bool boxed = !(MODE == M_PIRK_1A || MODE == M_PIRK_1B || MODE == M_PIRK3_2A || MODE == M_PIRK3_2B);
 };
 
-// RK4 stage 4 holds the most operands; asking for 4 waves per SIMD (a few
-// spilled registers) measured faster than the 3 its natural allocation gives.
 template <int MODE, int EF, int TJ>
 __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, const int sysidx) {
   constexpr bool N14 = ModeTraits<MODE>::n14;
@@ -85,6 +89,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
   constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
   constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
   constexpr bool BOXED = Combine<MODE>::boxed;
+  // RK4 stage 4 combines u_p and k3 at the point: the march already holds
+  // both as the raw operands of its stage input u_p + k3
+  constexpr bool KEEP = MODE == M_RK4_S4;
 
   constexpr int W = TI + 2 * HALO_R;
   __shared__ double sG[TJ][W];    // alpha*Pi
@@ -167,6 +174,41 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
     return E(f, q.o);
   };
   auto U = [&](int f, int i, int j, int k) { return Ur(f, remap(i, j, k)); };
+  // Deferred stage input.  fetch() issues the loads of field f at a resolved
+  // point and keeps the raw operands (u_p and k for the RK4 stage states);
+  // val() forms the value where it is used, so loads issued ahead of a
+  // barrier stay in flight across it.  Points resolved through the r-edge
+  // fills (xL extrapolation, inflow copy) are formed at once ("direct").
+  struct Raw {
+    double p, k;
+  };
+  // A zero the compiler cannot see through (dt * 0.0 is not folded without
+  // fast-math): the direct branch's unused k must not be a constant, or the
+  // stage arithmetic gets folded into the branches and the common path has
+  // to wait for its loads there.
+  const double opaque0 = G.dt * 0.0;
+  auto fetch = [&](int f, const Src& q, Raw& r) {
+    if constexpr (VIRT) {
+      if (q.kind == 0) {
+        r.p = gld(FP(R_IN, f), q.o + sysb);
+        r.k = gld(FP(R_KIN, f), q.o + sysb);
+      } else {
+        r.p = Ur(f, q);
+        r.k = opaque0;  // unused: val() selects p
+      }
+    } else {
+      r.p = gld(FP(R_IN, f), q.o + sysb);
+      r.k = opaque0;
+    }
+  };
+  auto val = [&](const Raw& r, bool direct) -> double {
+    if constexpr (VIRT) {
+      const double v = (MODE == M_RK4_S4) ? r.p + r.k : r.p + 0.50 * r.k;  // as E() forms it
+      return direct ? r.p : v;
+    } else {
+      return r.p;
+    }
+  };
 
   // own coefficients (the theta-ghost row of the tables is already the copy)
   const uint32_t own = valid ? (uint32_t)((jj * ld + ii) * 8) : 0u;
@@ -217,27 +259,31 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
     h3 = T(T_GRP, hown);
     h4 = T(T_SQRTG, hown);
   }
-  // halo values: Pi, psi_r and psi_theta (theta rows) or psi_phi (r columns)
-  auto load_halo = [&](int k, double& x0, double& x1, double& x2) {
+  // halo operands: Pi, psi_r and psi_theta (theta rows) or psi_phi (r columns)
+  auto fetch_halo = [&](int k, Raw* x, bool& direct) {
     if (hkind == 0) return;
     const Src q = remap(hi, hj, k);
+    direct = q.kind != 0;
     if (hkind == 1) {
       if constexpr (N14) {
-        x0 = Ur(4, q);
-        x1 = Ur(1, q);
+        fetch(4, q, x[0]);
+        fetch(1, q, x[1]);
       }
-      if constexpr (N5) x2 = Ur(2, q);
+      if constexpr (N5) fetch(2, q, x[2]);
     } else {
-      x0 = Ur(4, q);
-      x1 = Ur(1, q);
-      if constexpr (N5 && !EF) x2 = Ur(3, q);
+      fetch(4, q, x[0]);
+      fetch(1, q, x[1]);
+      if constexpr (N5 && !EF) fetch(3, q, x[2]);
     }
   };
 
   // phi queue at the own point: F and the phi-flux at k-1 and k
   double Pi = 0, psr = 0, pst = 0, psp = 0;  // own values at plane kk
   double Fm = 0, fpm = 0, F0 = 0, fp0 = 0;
-  double hx0 = 0, hx1 = 0, hx2 = 0;  // halo values at plane kk
+  Raw hr[3] = {{0, 0}, {0, 0}, {0, 0}};  // halo operands at plane kk
+  bool hdir = false;
+  Src qc{0u, 0};  // own point of plane kk, resolved
+  Raw cr[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // raw own operands of plane kk (KEEP)
   if (valid) {
     const int km = k0 == 0 ? Nz - 2 : k0 - 1;  // D_phi at k = 0 pairs f(Nz-2) with f(1)
     const Src qm = remap(ii, jj, km);
@@ -245,22 +291,29 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
     Fm = Fof(mPi, mpr);
     if constexpr (N5) fpm = fpof(mpr, Ur(3, qm));
     const Src q0 = remap(ii, jj, k0);
-    Pi = Ur(4, q0);
-    psr = Ur(1, q0);
+    qc = q0;
+    const bool d0 = q0.kind != 0;
+    fetch(4, q0, cr[0]);
+    fetch(1, q0, cr[1]);
+    Pi = val(cr[0], d0);
+    psr = val(cr[1], d0);
     if constexpr (N5) {
-      pst = Ur(2, q0);
-      psp = Ur(3, q0);
+      fetch(2, q0, cr[2]);
+      fetch(3, q0, cr[3]);
+      pst = val(cr[2], d0);
+      psp = val(cr[3], d0);
       fp0 = fpof(psr, psp);
     }
     F0 = Fof(Pi, psr);
   }
-  load_halo(k0, hx0, hx1, hx2);
+  fetch_halo(k0, hr, hdir);
 
   const double rr = G.rr, rt = G.rt, rp = G.rp, dt = G.dt;
   const int c = lane + HALO_R;  // own column in the r-arrays
 
   for (int kk = k0; kk < k1; ++kk) {
     const uint32_t sk = (uint32_t)kk * plane8;
+    const double hx0 = val(hr[0], hdir), hx1 = val(hr[1], hdir), hx2 = val(hr[2], hdir);
     // --- products of plane kk into LDS (from registers) ----------------------
     if (valid) {
       if constexpr (N14) {
@@ -294,27 +347,101 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
       }
     }
     // --- loads issued before the barrier: next plane, combine operands --------
-    double nPi = 0, npsr = 0, npst = 0, npsp = 0;
+    Raw nr[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // Pi, psi_r, psi_theta, psi_phi at kk+1
+    bool ndir = false;
+    Src qn{0u, 0};
     if (valid && kk < Nz) {
-      const Src qn = remap(ii, jj, kk + 1);
-      nPi = Ur(4, qn);
-      npsr = Ur(1, qn);
+      qn = remap(ii, jj, kk + 1);
+      ndir = qn.kind != 0;
+      fetch(4, qn, nr[0]);
+      fetch(1, qn, nr[1]);
       if constexpr (N5) {
-        npst = Ur(2, qn);
-        npsp = Ur(3, qn);
+        fetch(2, qn, nr[2]);
+        fetch(3, qn, nr[3]);
       }
     }
-    double nh0 = 0, nh1 = 0, nh2 = 0;
-    if (kk + 1 < k1) load_halo(kk + 1, nh0, nh1, nh2);
+    Raw nhr[3] = {{0, 0}, {0, 0}, {0, 0}};
+    bool nhdir = false;
+    if (kk + 1 < k1) fetch_halo(kk + 1, nhr, nhdir);
+    // phi at the own point for the mass term of rhs_pi
+    Raw vr{0, 0};
+    if constexpr (N5) {
+      if (act) fetch(0, qc, vr);
+    }
     const uint32_t so = sk + sysb + own;
     __syncthreads();
+    // keep the arithmetic on the operands above below the barrier
+    __builtin_amdgcn_sched_barrier(0);
+    double nPi = 0, npsr = 0, npst = 0, npsp = 0;  // plane kk+1, formed once its loads are needed
+    auto next_values = [&]() {
+      nPi = val(nr[0], ndir);
+      npsr = val(nr[1], ndir);
+      if constexpr (N5) {
+        npst = val(nr[2], ndir);
+        npsp = val(nr[3], ndir);
+      }
+    };
 
+    // combine operands and RHS terms of plane kk; vb, va, va2, vk: u_p / P / u,
+    // the PIRK3 stage state, RK4 k1, k2, k3 at the point
+    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+    double R[5] = {0, 0, 0, 0, 0};
+    double t1 = 0, t2 = 0, t5 = 0;
+    double vphi = 0;
+    const int kt = (kk + dis) % (Nz - 1);
+    // D_r / A_r on an LDS r-array (finite_differences.f90:36-42, :779-799)
+    auto Dr = [&](const double(*A)[W]) {
+      if constexpr (!BOXED) {
+        if (ii == 0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
+        if (ii == ni - 1) return (+3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+      }
+      return (-A[row][c - 1] + A[row][c + 1]) * rr;
+    };
+    auto Ar = [&](const double(*A)[W]) {
+      if (ii <= 1) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
+      if (ii == ni - 2) return (A[row][c + 1] - A[row][c - 1]) * rr;
+      if constexpr (!BOXED) {
+        if (ii == ni - 1) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+      }
+      if (be > 0.0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
+      if (be < 0.0) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
+      return (A[row][c + 1] - A[row][c - 1]) * rr;
+    };
+    // D_theta on an LDS theta-array (finite_differences.f90:214-230); the
+    // pole row reads plane k_target and the last row may reach below the
+    // tile, both from global memory through `far`.
+    auto Dth = [&](const double(*A)[TI], auto far) {
+      if constexpr (!BOXED) {
+        if (jj == 0) return (-A[row + 2][lane] + far(2, kt)) * rt;
+        if (jj == nj - 1) {
+          const double m2 = row >= 1 ? A[row - 1][lane] : far(nj - 3, kk);
+          return (+3.0 * A[row + 1][lane] - 4.0 * A[row][lane] + m2) * rt;
+        }
+      }
+      return (-A[row][lane] + A[row + 2][lane]) * rt;
+    };
+    // D_phi (finite_differences.f90:328-335); k = Nz pairs f(Nz-1) with f(2)
+    auto Dph = [&](double qm, double qp, auto at2) {
+      if constexpr (!BOXED) {
+        if (kk == Nz) return (-at2() + qm) * rp;
+      }
+      return (-qm + qp) * rp;
+    };
     if (act) {
-      // combine operands at the point (issued ahead of the RHS arithmetic)
-      double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+      // combine operands (issued ahead of the RHS arithmetic)
+      // KEEP: u_p and k3 at the point are the raw operands of the march
+      // (plane 0 resolves to plane Nz-1, so it reads plane 0 itself: the x and
+      // y fills read it before the z fill overwrites it)
+      const bool kept = KEEP && kk != 0;
       if constexpr (CBASE) {
+        if (kept) {
+          vb[0] = vr.p;
 #pragma unroll
-        for (int f = CLO; f < CHI; ++f) vb[f] = gld(FP(R_BASE, f), so);
+          for (int x = 0; x < 4; ++x) vb[x == 0 ? 4 : x] = cr[x].p;
+        } else {
+#pragma unroll
+          for (int f = CLO; f < CHI; ++f) vb[f] = gld(FP(R_BASE, f), so);
+        }
       }
       if constexpr (CAUX) {
 #pragma unroll
@@ -322,56 +449,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
       }
       if constexpr (MODE == M_RK4_S4) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) {
-          va2[f] = gld(FP(R_AUX2, f), so);
-          vk[f] = gld(FP(R_KIN, f), so);
+        for (int f = 0; f < 5; ++f) va2[f] = gld(FP(R_AUX2, f), so);
+        if (kept) {
+          vk[0] = vr.k;
+#pragma unroll
+          for (int x = 0; x < 4; ++x) vk[x == 0 ? 4 : x] = cr[x].k;
+        } else {
+#pragma unroll
+          for (int f = 0; f < 5; ++f) vk[f] = gld(FP(R_KIN, f), so);
         }
       }
-      double vphi = 0;
-      if constexpr (N5) vphi = U(0, ii, jj, kk);
-      const int kt = (kk + dis) % (Nz - 1);
-      const double Fp = Fof(nPi, npsr);
-      double fpp = 0;
-      if constexpr (N5) fpp = fpof(npsr, npsp);
-      double R[5] = {0, 0, 0, 0, 0};
-      // D_r / A_r on an LDS r-array (finite_differences.f90:36-42, :779-799)
-      auto Dr = [&](const double(*A)[W]) {
-        if constexpr (!BOXED) {
-          if (ii == 0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
-          if (ii == ni - 1) return (+3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
-        }
-        return (-A[row][c - 1] + A[row][c + 1]) * rr;
-      };
-      auto Ar = [&](const double(*A)[W]) {
-        if (ii <= 1) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
-        if (ii == ni - 2) return (A[row][c + 1] - A[row][c - 1]) * rr;
-        if constexpr (!BOXED) {
-          if (ii == ni - 1) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
-        }
-        if (be > 0.0) return (-3.0 * A[row][c] + 4.0 * A[row][c + 1] - A[row][c + 2]) * rr;
-        if (be < 0.0) return (3.0 * A[row][c] - 4.0 * A[row][c - 1] + A[row][c - 2]) * rr;
-        return (A[row][c + 1] - A[row][c - 1]) * rr;
-      };
-      // D_theta on an LDS theta-array (finite_differences.f90:214-230); the
-      // pole row reads plane k_target and the last row may reach below the
-      // tile, both from global memory through `far`.
-      auto Dth = [&](const double(*A)[TI], auto far) {
-        if constexpr (!BOXED) {
-          if (jj == 0) return (-A[row + 2][lane] + far(2, kt)) * rt;
-          if (jj == nj - 1) {
-            const double m2 = row >= 1 ? A[row - 1][lane] : far(nj - 3, kk);
-            return (+3.0 * A[row + 1][lane] - 4.0 * A[row][lane] + m2) * rt;
-          }
-        }
-        return (-A[row][lane] + A[row + 2][lane]) * rt;
-      };
-      // D_phi (finite_differences.f90:328-335); k = Nz pairs f(Nz-1) with f(2)
-      auto Dph = [&](double qm, double qp, auto at2) {
-        if constexpr (!BOXED) {
-          if (kk == Nz) return (-at2() + qm) * rp;
-        }
-        return (-qm + qp) * rp;
-      };
+      // terms from LDS and the own registers first: they need none of the
+      // loads issued before the barrier, which complete meanwhile
       if constexpr (N14) {
         R[0] = F0;  // rhs_phi = alpha*Pi + beta*psi_r (:675)
         R[1] = Dr(sG) + be * Ar(sPR) + dxb * psr;  // :687-689
@@ -379,17 +468,27 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
           const uint32_t o = (uint32_t)((j * ld + ii) * 8);
           return T(T_ALPHA, o) * U(4, ii, j, k) + T(T_BETA, o) * U(1, ii, j, k);
         });
-        R[3] = Dph(Fm, Fp, [&]() { return Fof(U(4, ii, jj, 2), U(1, ii, jj, 2)); });
       }
       if constexpr (N5) {
-        const double t1 = Dr(sFR) / sg;
-        const double t2 = Dth(sFT, [&](int j, int k) {
-                            return T(T_ASGTT, (uint32_t)((j * ld + ii) * 8)) * U(2, ii, j, k);
-                          }) / sg;
+        t1 = Dr(sFR) / sg;
+        t2 = Dth(sFT, [&](int j, int k) { return T(T_ASGTT, (uint32_t)((j * ld + ii) * 8)) * U(2, ii, j, k); }) /
+             sg;
+        t5 = be * Ar(sPS) / sg;
+      }
+    }
+    // the next plane's values, once the terms above have covered its loads
+    __builtin_amdgcn_sched_barrier(0);
+    next_values();
+    const double Fp = Fof(nPi, npsr);
+    double fpp = 0;
+    if constexpr (N5) fpp = fpof(npsr, npsp);
+    if (act) {
+      if constexpr (N5) vphi = val(vr, qc.kind != 0);
+      if constexpr (N14) R[3] = Dph(Fm, Fp, [&]() { return Fof(U(4, ii, jj, 2), U(1, ii, jj, 2)); });
+      if constexpr (N5) {
         const double dp = Dph(fpm, fpp, [&]() { return fpof(U(1, ii, jj, 2), U(3, ii, jj, 2)); });
         const double t3 = EF ? dp : dp / sg;
         const double t4 = Pi * dxb;
-        const double t5 = be * Ar(sPS) / sg;
         const double t6 = am2 * vphi;
         R[4] = t1 + t2 + t3 + t4 + t5 - t6;  // :728-736
       }
@@ -429,33 +528,34 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
 #pragma unroll
         for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, third * vb[f] + two_thirds * (cur[f] + dt * R[f]));
       }
-      Fm = F0;
-      fpm = fp0;
-      F0 = Fp;
-      fp0 = fpp;
-    } else {
-      Fm = F0;
-      fpm = fp0;
-      F0 = Fof(nPi, npsr);
-      if constexpr (N5) fp0 = fpof(npsr, npsp);
     }
+    Fm = F0;
+    fpm = fp0;
+    F0 = Fp;
+    fp0 = fpp;
     __syncthreads();
     Pi = nPi;
     psr = npsr;
     pst = npst;
     psp = npsp;
-
-    hx0 = nh0;
-    hx1 = nh1;
-    hx2 = nh2;
+    qc = qn;
+    if constexpr (KEEP) {
+#pragma unroll
+      for (int x = 0; x < 4; ++x) cr[x] = nr[x];
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) hr[x] = nhr[x];
+    hdir = nhdir;
   }
 }
 
-// One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The RK4
-// final stage asks for 16/TJ resident workgroups per CU: 4 waves per SIMD, so
-// its 20 input streams keep up to 128 VGPRs and never spill to scratch.
+// One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The
+// second launch bound is the minimum number of waves per SIMD.  RK4 stages 2
+// and 3 ask for 4 (128 VGPRs; 2 over their natural allocation).  The final
+// stage asks for 2 at TJ = 8 (no constraint: 3 waves with spills measured
+// slower than 2 without) and 4 at TJ = 4.
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : 1)) void stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel(const StageArgs a) {
   stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
 }
 
@@ -474,7 +574,7 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ start, int n, i
 }
 
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : 1)) void stage_kernel_batch(
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel_batch(
     const StageArgs* __restrict__ descs, const int* __restrict__ tstart, int ngrids, int total) {
   const int b = remap_xcd((int)blockIdx.x, total);
   const int q = find_grid(tstart, ngrids, b);
