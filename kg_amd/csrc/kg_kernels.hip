@@ -50,8 +50,8 @@ struct ModeTraits {
 
 __device__ __forceinline__ int remap_xcd(int b, int nb) {
   // Workgroup b runs on XCD b % NXCD.  Give each XCD a contiguous run of
-  // tiles so tiles adjacent in theta (which share halo rows) meet in the
-  // same L2.  Bijective on [0, nb).
+  // tiles so neighbouring tiles (which share halo planes and rows) meet in
+  // the same L2.  Bijective on [0, nb).
   const int q = nb / NXCD, r = nb % NXCD;
   const int x = b % NXCD, idx = b / NXCD;
   return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
@@ -102,9 +102,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
   __shared__ double sFT[TJ + 2][TI];  // theta-flux of rhs_pi
 
   const Geom& G = a.G;
+  // r tiles fastest, then phi chunks, then theta rows: tiles that follow each
+  // other on one XCD share the phi planes at their chunk edges in its L2
   const int ti = tile % a.nti;
-  const int tj = (tile / a.nti) % a.ntj;
-  const int tk = tile / (a.nti * a.ntj);
+  const int tk = (tile / a.nti) % a.ntk;
+  const int tj = tile / (a.nti * a.ntk);
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
   const int tid = row * TI + lane;
   const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
