@@ -102,11 +102,26 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
   __shared__ double sFT[TJ + 2][TI];  // theta-flux of rhs_pi
 
   const Geom& G = a.G;
-  // r tiles fastest, then phi chunks, then theta rows: tiles that follow each
-  // other on one XCD share the phi planes at their chunk edges in its L2
+  // r tiles fastest, then phi chunks of pairs of theta rows, then the pairs:
+  // tiles that follow each other on one XCD share the phi planes at their
+  // chunk edges and one theta halo row of the pair in its L2
   const int ti = tile % a.nti;
-  const int tk = (tile / a.nti) % a.ntk;
-  const int tj = tile / (a.nti * a.ntk);
+  int tj, tk;
+  {
+    constexpr int P = 2;
+    const int rest = tile / a.nti;
+    const int full = a.ntj / P;
+    const int tiles_full = full * P * a.ntk;
+    if (rest < tiles_full) {
+      const int g2 = rest / (P * a.ntk), r2 = rest % (P * a.ntk);
+      tj = g2 * P + r2 % P;
+      tk = r2 / P;
+    } else {  // an odd last theta row
+      const int r3 = rest - tiles_full, rem = a.ntj - full * P;
+      tj = full * P + r3 % rem;
+      tk = r3 / rem;
+    }
+  }
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
   const int tid = row * TI + lane;
   const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
