@@ -201,6 +201,26 @@ kg_status kg_spec_radial(kg_spec* p, int32_t field, int32_t precision, double* f
 /* The radial contraction alone, on caller-supplied host flm (complex
  * [nlm][Nx]) -> host ft (complex [nlm][nk]). */
 kg_status kg_spec_radial_flm(kg_spec* p, const double* flm, int32_t precision, double* ft);
+/* Detector diagnostics at spectral radii index_det[0..n_det) (the
+ * reference's save-step diagnostics, diagnostic_lib.f90:197-199):
+ * momentum = complex [n_det][3], the fluxes trapezium_2D(domain_to_sph(
+ * j_i sqrtg)) of currents_ADM / currents_detectors (:214-224,250-274);
+ * alm = complex [n_det][nlm], project_sh of domain_to_sph(phi)
+ * (proyector_detectors, :231-248).  The reference never assigns index_det;
+ * callers pass the radius indices.  Either output may be NULL. */
+kg_status kg_spec_detectors(kg_spec* p, int32_t n_det, const int32_t* index_det, double* momentum, double* alm);
+
+/* ------------------------------------------------------------------------
+ * Volume integral of the state (integrate, integral.f90:646-669, with
+ * trapezium_real :220-255 over ii in [g, ni-1-g], jj in [g, nj-1-g], all
+ * phi planes).  quantity 0..4: that field (complex for a complex state);
+ * KG_DIAG_DENSITY: density() (diagnostic_lib.f90:258-271) times sqrt(gamma),
+ * the integrand of Mass_scalar (:207).  out[0] = real part, out[1] =
+ * imaginary part.  Synchronous.
+ * ------------------------------------------------------------------------ */
+enum { KG_DIAG_DENSITY = 5 };
+kg_status kg_diag_integrate(kg_handle* h, int32_t quantity, double out[2]);
+const char* kg_diag_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,7 @@ class KGDesc(C.Structure):
 
 KG_BASIS_ZEROS, KG_BASIS_UNIFORM = range(2)
 KG_SPEC_FP64, KG_SPEC_SPLIT_BF16 = range(2)
+KG_DIAG_DENSITY = 5
 
 
 class KGSpecDesc(C.Structure):
@@ -76,6 +77,9 @@ SIGNATURES = {
     "kg_spec_project_sh": (C.c_int32, [_vp, C.c_int32, _vp, C.c_int32]),
     "kg_spec_radial": (C.c_int32, [_vp, C.c_int32, C.c_int32, _vp, C.c_int32]),
     "kg_spec_radial_flm": (C.c_int32, [_vp, _vp, C.c_int32, _vp]),
+    "kg_spec_detectors": (C.c_int32, [_vp, C.c_int32, _vp, _vp, _vp]),
+    "kg_diag_integrate": (C.c_int32, [_vp, C.c_int32, _dp]),
+    "kg_diag_last_error": (C.c_char_p, []),
 }
 
 

@@ -828,6 +828,8 @@ int handle_view(const kg_handle* h, HandleView* v) {
   v->hg = h->hg;
   v->nsys = h->nsys;
   for (int f = 0; f < kNumFields; ++f) v->u[f] = h->u.f[f];
+  v->tab = h->d_tab;
+  v->m = h->desc.m;
   return 0;
 }
 }  // namespace kg

@@ -19,7 +19,7 @@ struct kg_handle;
 namespace kg {
 
 constexpr int kNumFields = 5;  // phi, psi_r, psi_theta, psi_phi, Pi
-constexpr int kNumTables = 10;
+constexpr int kNumTables = 11;
 
 // table slots
 enum Table : int {
@@ -32,7 +32,8 @@ enum Table : int {
   T_GRP = 6,     // gamma^{r phi}
   T_ASGTT = 7,   // (alpha*sqrtg)*gamma^{theta theta}
   T_GPP = 8,     // gamma^{phi phi}
-  T_AM2 = 9      // alpha*m**2
+  T_AM2 = 9,     // alpha*m**2
+  T_GTT = 10     // gamma^{theta theta} (diagnostics: currents_ADM, density)
 };
 
 struct Geom {
@@ -138,6 +139,8 @@ struct HandleView {
   HostGrid hg;
   int nsys;
   double* u[5];
+  const double* tab;  // kNumTables metric tables [nj][ld]
+  double m;           // field mass
 };
 int handle_view(const ::kg_handle* h, HandleView* v);
 

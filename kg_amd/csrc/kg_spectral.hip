@@ -395,6 +395,131 @@ __global__ __launch_bounds__(256) void radial_mfma_kernel(const int4* __restrict
   }
 }
 
+// ---------------------------------------------------------------- detectors
+// currents_ADM (diagnostic_lib.f90:214-224) times sqrt(gamma) at a padded
+// point: j1 = -Pi (g11 psi_r + g13 psi_phi), j2 = -Pi g22 psi_th,
+// j3 = -Pi (g13 psi_r + g33 psi_phi), complex products (ac - bd, ad + bc).
+struct CurArgs {
+  const double* u[5];
+  const double* tab;
+  int64_t sys_stride, plane, n2;
+  int ld, nsys;
+};
+
+__device__ void current_at(const CurArgs& c, int ii, int jj, int kk, double* v /* re0 re1 re2 im0 im1 im2 */) {
+  const int64_t o = (int64_t)kk * c.plane + (int64_t)jj * c.ld + ii;
+  const int64_t t = (int64_t)jj * c.ld + ii;
+  const bool cp = c.nsys == 2;
+  const double pr = c.u[4][o], pi = cp ? c.u[4][o + c.sys_stride] : 0.0;
+  const double rr = c.u[1][o], ri = cp ? c.u[1][o + c.sys_stride] : 0.0;
+  const double tr = c.u[2][o], ti = cp ? c.u[2][o + c.sys_stride] : 0.0;
+  const double fr = c.u[3][o], fi = cp ? c.u[3][o + c.sys_stride] : 0.0;
+  const double g11 = c.tab[T_GRR * c.n2 + t], g13 = c.tab[T_GRP * c.n2 + t];
+  const double g22 = c.tab[T_GTT * c.n2 + t], g33 = c.tab[T_GPP * c.n2 + t];
+  const double sg = c.tab[T_SQRTG * c.n2 + t];
+  const double c1r = g11 * rr + g13 * fr, c1i = g11 * ri + g13 * fi;
+  const double c3r = g13 * rr + g33 * fr, c3i = g13 * ri + g33 * fi;
+  const double q2r = pr * g22, q2i = pi * g22;
+  const double j1r = -(pr * c1r - pi * c1i), j1i = -(pr * c1i + pi * c1r);
+  const double j2r = -(q2r * tr - q2i * ti), j2i = -(q2r * ti + q2i * tr);
+  const double j3r = -(pr * c3r - pi * c3i), j3i = -(pr * c3i + pi * c3r);
+  // j_i(c) * sqrtg: complex times real
+  v[0] = j1r * sg;
+  v[1] = j2r * sg;
+  v[2] = j3r * sg;
+  v[3] = j1i * sg;
+  v[4] = j2i * sg;
+  v[5] = j3i * sg;
+}
+
+// domain_to_sph of the six components at spectral point (i, j, k): the
+// arithmetic of interp() on values formed at the corners
+__device__ void interp_current(const InterpArgs& a, const CurArgs& c, int i, int j, int k, double* out) {
+  const int ir = a.ir[i];
+  const int tp = a.tpole[j];
+  const int kp = a.iph[k];
+  if (tp == 1 || tp == 2) {
+    current_at(c, ir, tp == 1 ? 0 : a.nj - 1, kp, out);
+    return;
+  }
+  if (a.pskip[k]) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) out[q] = 0.0;
+    return;
+  }
+  const int jt = a.ith[j];
+  const int k1 = kp + 1;
+  double f000[6], f001[6], f010[6], f011[6], f100[6], f101[6], f110[6], f111[6];
+  current_at(c, ir, jt, kp, f000);
+  current_at(c, ir, jt, k1, f001);
+  current_at(c, ir, jt + 1, kp, f010);
+  current_at(c, ir, jt + 1, k1, f011);
+  current_at(c, ir + 1, jt, kp, f100);
+  current_at(c, ir + 1, jt, k1, f101);
+  current_at(c, ir + 1, jt + 1, kp, f110);
+  current_at(c, ir + 1, jt + 1, k1, f111);
+  const double w_r = a.wr[i], w_th = a.wth[j], w_ph = a.wph[k];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double f00 = (1.0 - w_ph) * f000[q] + w_ph * f001[q];
+    const double f01 = (1.0 - w_ph) * f010[q] + w_ph * f011[q];
+    const double f10 = (1.0 - w_ph) * f100[q] + w_ph * f101[q];
+    const double f11 = (1.0 - w_ph) * f110[q] + w_ph * f111[q];
+    const double f0 = (1.0 - w_th) * f00 + w_th * f01;
+    const double f1 = (1.0 - w_th) * f10 + w_th * f11;
+    out[q] = (1.0 - w_r) * f0 + w_r * f1;
+  }
+}
+
+// currents_detectors (diagnostic_lib.f90:250-274): one block per detector
+// radius, trapezium_2D (integral.f90:330-362) of the interpolated j_i sqrtg
+// over the sphere; mom[d][c] complex.
+__global__ __launch_bounds__(256) void detector_flux_kernel(const InterpArgs a, const CurArgs c,
+                                                            const int* __restrict__ idx, double dth, double dph,
+                                                            double* __restrict__ mom) {
+  __shared__ double red[6][256];
+  const int d = blockIdx.x;
+  const int i = idx[d];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int n = a.Nt * a.Np;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int j = e / a.Np, k = e % a.Np;
+    double v[6];
+    interp_current(a, c, i, j, k, v);
+    const int edge = (j == 0 || j == a.Nt - 1) + 2 * (k == 0 || k == a.Np - 1);
+    const double w = edge == 0 ? 1.0 : (edge == 3 ? 0.25 : 0.5);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] += w * v[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s2];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      mom[(d * 3 + cc) * 2 + 0] = dth * dph * red[cc][0];
+      mom[(d * 3 + cc) * 2 + 1] = dth * dph * red[3 + cc][0];
+    }
+  }
+}
+
+// alm[d][lm] (complex) = flm at detector radius idx[d]
+__global__ void gather_alm_kernel(const double* __restrict__ flm, const int* __restrict__ idx, int ndet, int nlm,
+                                  int Nr, double* __restrict__ alm) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= ndet * nlm) return;
+  const int d = e / nlm, lm = e % nlm;
+  alm[2 * e] = flm[((int64_t)lm * 2 + 0) * Nr + idx[d]];
+  alm[2 * e + 1] = flm[((int64_t)lm * 2 + 1) * Nr + idx[d]];
+}
+
 }  // namespace
 
 // ===================================================================== host
@@ -878,4 +1003,57 @@ kg_status kg_spec_radial_flm(kg_spec* p, const double* flm, int32_t precision, d
   st = enqueue_radial(p, v.stream, precision);
   if (st != KG_OK) return st;
   return deliver(p, v.stream, p->d_ft, ft, (size_t)p->nlm * p->nk * 2, 0);
+}
+
+kg_status kg_spec_detectors(kg_spec* p, int32_t n_det, const int32_t* index_det, double* momentum, double* alm) {
+  if (!p || !index_det || n_det < 1) return sfail(p, KG_ERR_INVALID, "need a projector and n_det >= 1 indices");
+  for (int d = 0; d < n_det; ++d)
+    if (index_det[d] < 0 || index_det[d] >= p->Nr)
+      return sfail(p, KG_ERR_INVALID, "index_det[%d] = %d outside 0..%d", d, index_det[d], p->Nr - 1);
+  HandleView v;
+  kg_status st = view(p, &v);
+  if (st != KG_OK) return st;
+  int* d_idx = nullptr;
+  double* d_out = nullptr;
+  const size_t nmom = (size_t)n_det * 3 * 2, nalm = (size_t)n_det * p->nlm * 2;
+  if (dalloc(p, &d_idx, (size_t)n_det) != KG_OK) return KG_ERR_NOMEM;
+  if (dalloc(p, &d_out, nmom + nalm) != KG_OK) {
+    (void)hipFree(d_idx);
+    return KG_ERR_NOMEM;
+  }
+  hipError_t e = hipMemcpyAsync(d_idx, index_det, sizeof(int) * n_det, hipMemcpyHostToDevice, v.stream);
+  if (e == hipSuccess && momentum) {
+    CurArgs c;
+    for (int f = 0; f < 5; ++f) c.u[f] = v.u[f];
+    c.tab = v.tab;
+    c.sys_stride = v.G.sys_stride;
+    c.plane = v.G.plane;
+    c.n2 = (int64_t)v.G.nj * v.G.ld;
+    c.ld = v.G.ld;
+    c.nsys = v.nsys;
+    hipLaunchKernelGGL(detector_flux_kernel, dim3(n_det), dim3(256), 0, v.stream, p->ia, c, d_idx, v.hg.dy, v.hg.dz,
+                       d_out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && alm) {
+    st = enqueue_phi(p, v, 0);
+    if (st == KG_OK) st = enqueue_theta(p, v);
+    if (st != KG_OK) {
+      (void)hipStreamSynchronize(v.stream);
+      (void)hipFree(d_idx);
+      (void)hipFree(d_out);
+      return st;
+    }
+    const int n = n_det * p->nlm;
+    hipLaunchKernelGGL(gather_alm_kernel, dim3((n + 255) / 256), dim3(256), 0, v.stream, p->d_flm, d_idx, n_det,
+                       p->nlm, p->Nr, d_out + nmom);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+  if (e == hipSuccess && momentum) e = hipMemcpy(momentum, d_out, nmom * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && alm) e = hipMemcpy(alm, d_out + nmom, nalm * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d_idx);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return sfail(p, KG_ERR_HIP, "detectors: %s", hipGetErrorString(e));
+  return KG_OK;
 }

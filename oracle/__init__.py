@@ -338,3 +338,56 @@ def domain_to_sph(fd, Nx, Ny, Nz, g, xmin, dx, dy, dz):
     out = np.zeros((Nz, Ny, Nx))
     _spec().kgs_domain_to_sph(Nx, Ny, Nz, g, xmin, dx, dy, dz, _ptr(fd), _ptr(out))
     return out
+
+
+# --------------------------------------------------------------------------
+# Diagnostics (kg_diag_oracle.c)
+# --------------------------------------------------------------------------
+def _diag():
+    L = lib()
+    if not getattr(L, "_diag_ready", False):
+        L.kgd_density.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.kgd_currents.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.kgd_integrate.restype = C.c_double
+        L.kgd_integrate.argtypes = [C.c_void_p, _dp]
+        L.kgd_trapezium_2d_complex.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L._diag_ready = True
+    return L
+
+
+def _opt(a):
+    return _ptr(a) if a is not None else None
+
+
+def density(sim, u_re, u_im=None):
+    """density() (diagnostic_lib.f90:258-271) of a padded state [5][nk][nj][ni]."""
+    u_re = np.ascontiguousarray(u_re, dtype=np.float64)
+    u_im = None if u_im is None else np.ascontiguousarray(u_im, dtype=np.float64)
+    rho = np.zeros(u_re.shape[1:])
+    _diag().kgd_density(sim._h, _ptr(u_re), _opt(u_im), _ptr(rho))
+    return rho
+
+
+def currents(sim, u_re, u_im=None):
+    """currents_ADM (diagnostic_lib.f90:214-224): complex j[3][nk][nj][ni]."""
+    u_re = np.ascontiguousarray(u_re, dtype=np.float64)
+    u_im = None if u_im is None else np.ascontiguousarray(u_im, dtype=np.float64)
+    jr = np.zeros((3,) + u_re.shape[1:])
+    ji = np.zeros_like(jr)
+    _diag().kgd_currents(sim._h, _ptr(u_re), _opt(u_im), _ptr(jr), _ptr(ji))
+    return jr + 1j * ji
+
+
+def integrate(sim, f):
+    """integrate_real (integral.f90:646-669) of a padded real field [nk][nj][ni]."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return _diag().kgd_integrate(sim._h, _ptr(f))
+
+
+def trapezium_2d_complex(f, dx, dy):
+    """trapezium_2D_complex (integral.f90:330-362) of f[i][j]."""
+    f = np.asarray(f, dtype=np.complex128)
+    fr, fi = np.ascontiguousarray(f.real), np.ascontiguousarray(f.imag)
+    o_r, o_i = C.c_double(), C.c_double()
+    _diag().kgd_trapezium_2d_complex(f.shape[0], f.shape[1], dx, dy, _ptr(fr), _ptr(fi), C.byref(o_r), C.byref(o_i))
+    return complex(o_r.value, o_i.value)
