@@ -110,6 +110,15 @@ kg_status kg_download_aux(kg_handle* h, int32_t which, double* u, int32_t layout
  * keep data in HBM).  dir = 0: src -> handle; dir = 1: handle -> dst. */
 kg_status kg_copy_state_device(kg_handle* h, void* dev_ptr, int32_t dir);
 
+/* check_point / read_from_check_point (lines_method_lib.f90:599-667): the
+ * state, its time and, for PIRK2/PIRK3, the stage arrays whose Pi carries
+ * over between steps, in a binary file of this engine (the reference writes
+ * list-directed text and its reader deletes the file before reading it).
+ * Loading requires a handle created with the same descriptor; a restarted run
+ * continues bit for bit as if it had not stopped. */
+kg_status kg_checkpoint_save(kg_handle* h, const char* path);
+kg_status kg_checkpoint_load(kg_handle* h, const char* path);
+
 /* boundary() of the integrators (lines_method_lib.f90:313-330): xL, xR, yL,
  * yR, zL, zR on MoL%u, as the initial-data code calls them
  * (initial_conditions.f90:111-121). */
@@ -221,6 +230,12 @@ kg_status kg_spec_detectors(kg_spec* p, int32_t n_det, const int32_t* index_det,
 enum { KG_DIAG_DENSITY = 5 };
 kg_status kg_diag_integrate(kg_handle* h, int32_t quantity, double out[2]);
 const char* kg_diag_last_error(void);
+/* The arrays the reference writes to HDF5 every save step
+ * (save_gnuplot.f90:121-144), formed on the GPU over its slab i = -g..Nx,
+ * j = 0..Ny, k = 0..Nz-1: quantity 0 |phi|, 1 density(), 2 |psi_r|,
+ * 3 |psi_theta|, 4 |psi_phi|, 5 |Pi|, 6 Re phi, 7 Im phi.  out = host array
+ * [Nz][Ny+1][Nx+g+1] (i fastest).  Synchronous. */
+kg_status kg_diag_output(kg_handle* h, int32_t quantity, double* out);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,8 @@ SIGNATURES = {
     "kg_download_aux": (C.c_int32, [_vp, C.c_int32, _vp, C.c_int32]),
     "kg_copy_state_device": (C.c_int32, [_vp, _vp, C.c_int32]),
     "kg_fill_ghosts": (C.c_int32, [_vp]),
+    "kg_checkpoint_save": (C.c_int32, [_vp, C.c_char_p]),
+    "kg_checkpoint_load": (C.c_int32, [_vp, C.c_char_p]),
     "kg_pirk2": (C.c_int32, [_vp]),
     "kg_pirk3": (C.c_int32, [_vp]),
     "kg_rk3": (C.c_int32, [_vp]),
@@ -80,6 +82,7 @@ SIGNATURES = {
     "kg_spec_detectors": (C.c_int32, [_vp, C.c_int32, _vp, _vp, _vp]),
     "kg_diag_integrate": (C.c_int32, [_vp, C.c_int32, _dp]),
     "kg_diag_last_error": (C.c_char_p, []),
+    "kg_diag_output": (C.c_int32, [_vp, C.c_int32, _dp]),
 }
 
 

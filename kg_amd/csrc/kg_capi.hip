@@ -751,6 +751,97 @@ kg_status kg_copy_state_device(kg_handle* h, void* dev_ptr, int32_t dir) {
   return KG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Checkpoint / restart (the role of check_point / read_from_check_point,
+// lines_method_lib.f90:599-667, in a binary format of this engine; the
+// reference's reader deletes its file before reading it).  Layout:
+//   "KGCKPT01", kg_desc, double t, int32 nsys, int32 narrays, int64 count,
+//   then narrays x count doubles in KG_LAYOUT_SOA order: u, and for PIRK2 /
+//   PIRK3 the stage arrays u_stage (/ u_stage_2), whose Pi carries over
+//   between steps.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr char kCkptMagic[8] = {'K', 'G', 'C', 'K', 'P', 'T', '0', '1'};
+
+int aux_count(int scheme) { return scheme == KG_PIRK2 ? 1 : scheme == KG_PIRK3 ? 2 : 0; }
+
+bool same_grid(const kg_desc& a, const kg_desc& b) {
+  return a.nvars == b.nvars && a.g_pts == b.g_pts && a.Nx == b.Nx && a.Ny == b.Ny && a.Nz == b.Nz &&
+         a.scheme == b.scheme && a.metric == b.metric && a.bc_rmax == b.bc_rmax && a.is_complex == b.is_complex &&
+         a.xmin == b.xmin && a.xmax == b.xmax && a.dt == b.dt && a.a == b.a && a.M_bh == b.M_bh && a.m == b.m;
+}
+}  // namespace
+
+kg_status kg_checkpoint_save(kg_handle* h, const char* path) {
+  if (!h || !path) return fail(h, KG_ERR_INVALID, "null argument");
+  const int naux = aux_count(h->desc.scheme);
+  const int64_t count = (int64_t)h->G.ni * h->G.nj * h->G.nk * 5 * h->nsys;
+  std::vector<double> buf((size_t)count);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(h, KG_ERR_INVALID, "cannot open %s for writing", path);
+  const int32_t meta[2] = {h->nsys, 1 + naux};
+  bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8 && std::fwrite(&h->desc, sizeof(kg_desc), 1, f) == 1 &&
+            std::fwrite(&h->t, sizeof(double), 1, f) == 1 && std::fwrite(meta, sizeof(int32_t), 2, f) == 2 &&
+            std::fwrite(&count, sizeof(int64_t), 1, f) == 1;
+  kg_status st = KG_OK;
+  for (int a = 0; ok && st == KG_OK && a <= naux; ++a) {
+    if (a == 0) {
+      st = xfer(h, h->u, buf.data(), KG_LAYOUT_SOA, 0);
+    } else {
+      State5 s;
+      st = aux_state(h, a - 1, &s);
+      if (st == KG_OK) st = xfer(h, s, buf.data(), KG_LAYOUT_SOA, 0);
+    }
+    if (st == KG_OK) ok = std::fwrite(buf.data(), sizeof(double), (size_t)count, f) == (size_t)count;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (st != KG_OK) return st;
+  if (!ok) return fail(h, KG_ERR_INVALID, "write to %s failed", path);
+  return KG_OK;
+}
+
+kg_status kg_checkpoint_load(kg_handle* h, const char* path) {
+  if (!h || !path) return fail(h, KG_ERR_INVALID, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(h, KG_ERR_INVALID, "cannot open %s", path);
+  char magic[8];
+  kg_desc d{};
+  double t = 0.0;
+  int32_t meta[2] = {0, 0};
+  int64_t count = 0;
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 &&
+            std::fread(&d, sizeof(kg_desc), 1, f) == 1 && std::fread(&t, sizeof(double), 1, f) == 1 &&
+            std::fread(meta, sizeof(int32_t), 2, f) == 2 && std::fread(&count, sizeof(int64_t), 1, f) == 1;
+  const int naux = aux_count(h->desc.scheme);
+  const int64_t want = (int64_t)h->G.ni * h->G.nj * h->G.nk * 5 * h->nsys;
+  if (!ok) {
+    std::fclose(f);
+    return fail(h, KG_ERR_INVALID, "%s is not a checkpoint of this engine", path);
+  }
+  if (!same_grid(d, h->desc) || meta[0] != h->nsys || meta[1] != 1 + naux || count != want) {
+    std::fclose(f);
+    return fail(h, KG_ERR_INVALID, "%s holds a different grid, scheme or physics", path);
+  }
+  std::vector<double> buf((size_t)count);
+  kg_status st = KG_OK;
+  for (int a = 0; st == KG_OK && a <= naux; ++a) {
+    if (std::fread(buf.data(), sizeof(double), (size_t)count, f) != (size_t)count) {
+      st = fail(h, KG_ERR_INVALID, "%s is truncated", path);
+      break;
+    }
+    if (a == 0) {
+      st = xfer(h, h->u, buf.data(), KG_LAYOUT_SOA, 1);
+    } else {
+      State5 s;
+      st = aux_state(h, a - 1, &s);
+      if (st == KG_OK) st = xfer(h, s, buf.data(), KG_LAYOUT_SOA, 1);
+    }
+  }
+  std::fclose(f);
+  if (st == KG_OK) h->t = t;
+  return st;
+}
+
 kg_status kg_fill_ghosts(kg_handle* h) {
   if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
   return fill(h, h->u, h->u);

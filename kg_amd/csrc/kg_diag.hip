@@ -122,6 +122,34 @@ __global__ __launch_bounds__(256) void integrate_sum(const double* __restrict__ 
   }
 }
 
+// Output quantities over the slab the reference writes (save_gnuplot.f90:
+// 121-144: ii in [0, Nx+g] (i = -g..Nx), jj in [g, g+Ny], kk in [0, Nz-1]):
+// 0 |phi|, 1 density(), 2 |psi_r|, 3 |psi_theta|, 4 |psi_phi|, 5 |Pi|,
+// 6 Re phi, 7 Im phi.  out[k][j][i], i fastest (a Fortran-ordered
+// (i, j, k) array, as the reference's HDF5 datasets hold it).
+__global__ __launch_bounds__(256) void output_slab(const IntArgs a, int quantity, double* __restrict__ out) {
+  const int ni_o = a.Nx + a.g + 1, nj_o = a.Ny + 1;
+  const int ir = blockIdx.x * 256 + threadIdx.x;
+  const int jr = blockIdx.y, kk = blockIdx.z;
+  if (ir >= ni_o) return;
+  const int jj = a.g + jr, ii = ir;
+  const uint32_t t = (uint32_t)(jj * a.ld + ii);
+  const int64_t o = (int64_t)kk * a.plane + t;
+  const bool cplx = a.nsys == 2;
+  double v;
+  if (quantity == 1) {
+    v = density(a, o, t, cplx);
+  } else if (quantity == 6) {
+    v = a.u[0][o];
+  } else if (quantity == 7) {
+    v = cplx ? a.u[0][o + a.sys_stride] : 0.0;
+  } else {
+    const int f = quantity == 0 ? 0 : quantity - 1;
+    v = mag(a.u[f][o], cplx ? a.u[f][o + a.sys_stride] : 0.0, cplx);
+  }
+  out[((int64_t)kk * nj_o + jr) * ni_o + ir] = v;
+}
+
 thread_local std::string g_diag_error;
 
 kg_status dfail(kg_status st, const char* msg, hipError_t e = hipSuccess) {
@@ -171,5 +199,40 @@ kg_status kg_diag_integrate(kg_handle* h, int32_t quantity, double out[2]) {
   if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
   (void)hipFree(buf);
   if (e != hipSuccess) return dfail(KG_ERR_HIP, "integrate", e);
+  return KG_OK;
+}
+
+kg_status kg_diag_output(kg_handle* h, int32_t quantity, double* out) {
+  if (!h || !out) return dfail(KG_ERR_INVALID, "null argument");
+  if (quantity < 0 || quantity > 7) return dfail(KG_ERR_INVALID, "quantity must be 0..7");
+  HandleView v;
+  if (handle_view(h, &v) != 0) return dfail(KG_ERR_INVALID, "bad handle");
+  hipError_t e = hipSetDevice(v.device);
+  if (e != hipSuccess) return dfail(KG_ERR_HIP, "hipSetDevice", e);
+  IntArgs a;
+  for (int f = 0; f < 5; ++f) a.u[f] = v.u[f];
+  a.tab = v.tab;
+  a.sys_stride = v.G.sys_stride;
+  a.plane = v.G.plane;
+  a.ld = v.G.ld;
+  a.nj = v.G.nj;
+  a.g = v.G.g;
+  a.Nx = v.G.Nx;
+  a.Ny = v.G.Ny;
+  a.Nz = v.G.Nz;
+  a.nsys = v.nsys;
+  a.quantity = quantity;
+  a.m2 = v.m * v.m;
+  const int ni_o = a.Nx + a.g + 1, nj_o = a.Ny + 1, nk_o = a.Nz;
+  const size_t n = (size_t)ni_o * nj_o * nk_o;
+  double* buf = nullptr;
+  e = hipMalloc((void**)&buf, n * sizeof(double));
+  if (e != hipSuccess) return dfail(KG_ERR_NOMEM, "hipMalloc", e);
+  hipLaunchKernelGGL(output_slab, dim3((ni_o + 255) / 256, nj_o, nk_o), dim3(256), 0, v.stream, a, quantity, buf);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, buf, n * sizeof(double), hipMemcpyDeviceToHost, v.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return dfail(KG_ERR_HIP, "output", e);
   return KG_OK;
 }

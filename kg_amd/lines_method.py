@@ -156,6 +156,15 @@ class LinesMethod:
         """Exchange the state with a device buffer in SoA order (e.g. a torch CUDA tensor's data_ptr())."""
         capi.check(self._lib.kg_copy_state_device(self._h, C.c_void_p(dev_ptr), 0 if to_handle else 1), self._h)
 
+    # -- checkpoint / restart (lines_method_lib.f90:599-667) ---------------------
+    def check_point(self, path: str) -> None:
+        """Write the state, time and (PIRK) stage arrays to `path`."""
+        capi.check(self._lib.kg_checkpoint_save(self._h, str(path).encode()), self._h)
+
+    def read_from_check_point(self, path: str) -> None:
+        """Restore what check_point wrote (same grid, scheme and physics)."""
+        capi.check(self._lib.kg_checkpoint_load(self._h, str(path).encode()), self._h)
+
     # -- the path ---------------------------------------------------------------
     def boundary(self) -> None:
         """xL, xR, yL, yR, zL, zR fills of u (lines_method_lib.f90:313-330)."""
