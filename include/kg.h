@@ -237,6 +237,53 @@ const char* kg_diag_last_error(void);
  * [Nz][Ny+1][Nx+g+1] (i fastest).  Synchronous. */
 kg_status kg_diag_output(kg_handle* h, int32_t quantity, double* out);
 
+/* ------------------------------------------------------------------------
+ * Initial data built on the GPU: the reference's `subroutine
+ * initial_conditions` (main/initial_conditions.f90:1-220), selected there by
+ * initial_conditions_type with the globals mu and sigma.
+ *  KG_IC_SPHERICAL_SHELL  'spherical_shell' (:23-88, every metric but MKS):
+ *      phi = 1/x exp(-(x-mu)^2/sigma^2), psi_r = 0 (its assignment is
+ *      commented out there), psi_theta, psi_phi its stencil derivatives,
+ *      Pi = 1/alpha (dtphi - beta^r psi_r); Amp = 1/sqrt(integrate(rho alpha
+ *      sqrt(gamma))).  The MKS branch scales by an unset local (:26) and is
+ *      refused with KG_ERR_UNSUPPORTED.
+ *  KG_IC_RANDOM           'random' (:90-144): the Cartesian random field
+ *      (tools/random.f90:17-59) sampled onto the grid, xL/xR fills,
+ *      derivatives, Pi with dtphi = 0; Amp = 1/sqrt(integrate(m |phi|^2
+ *      sqrt(gamma))).
+ *  KG_IC_RANDOM_SHELL     'random_shell' (:146-188): the sampled field times
+ *      the shell envelope, derivatives, Pi with dtphi = psi_r, then the xL,
+ *      xR, zL, zR, yL, yR fills; Amp as for 'random'.
+ * The random types need a complex handle (is_complex = 1), as the reference's
+ * state is complex.  The random field lives on ncart^3 points over
+ * [-(r_max+1), r_max+1)^3 with r_max = x(iR+g) (the reference fixes
+ * ncart = 256); every grid point must fall inside the box's cells, i.e.
+ * r_max + 1 < ncart/2.  The random phases U in [0,1) (random.f90:47-48, an
+ * unseeded random_number there) come from `uniforms` when it is not NULL
+ * (host array of ncart^3 values, i fastest: Fortran's random_value(i,j,k))
+ * and otherwise from a SplitMix64 stream keyed by `seed`.  An 'inflow' xR
+ * fill reads the start-of-step state, which the reference has not written at
+ * this point; it is taken as zero.  *amp_out (may be NULL) receives Amp.
+ * The time is left unchanged.  Synchronous.
+ * ------------------------------------------------------------------------ */
+enum { KG_IC_SPHERICAL_SHELL = 0, KG_IC_RANDOM = 1, KG_IC_RANDOM_SHELL = 2 };
+typedef struct kg_ic_desc {
+  int32_t type;            /* KG_IC_* */
+  int32_t ncart;           /* random types: Cartesian points per axis, a power of two in [4, 1024] */
+  double mu, sigma;        /* shell centre and width (global_numbers) */
+  double p0;               /* random types: momentum width (the reference passes 1) */
+  uint64_t seed;           /* random types: phase stream when uniforms is NULL */
+  const double* uniforms;  /* random types: optional host array [ncart^3] */
+} kg_ic_desc;
+kg_status kg_initial_data(kg_handle* h, const kg_ic_desc* d, double* amp_out);
+/* The Cartesian field psi_x = ifft_3D(psi_p) of random.f90:17-59 alone, on
+ * device `device`: out = host complex(8) array psi_x(ncart,ncart,ncart) in
+ * Fortran order (re, im interleaved, i fastest); r_max as the reference
+ * passes it (the box is [-(r_max+1), r_max+1)).  Synchronous. */
+kg_status kg_ic_random_field(int32_t device, int32_t ncart, double r_max, double p0, uint64_t seed,
+                             const double* uniforms, double* out);
+const char* kg_ic_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,17 @@ class KGSpecDesc(C.Structure):
     ]
 
 
+KG_IC_SPHERICAL_SHELL, KG_IC_RANDOM, KG_IC_RANDOM_SHELL = range(3)
+
+
+class KGICDesc(C.Structure):
+    """Mirror of struct kg_ic_desc."""
+    _fields_ = [
+        ("type", C.c_int32), ("ncart", C.c_int32), ("mu", C.c_double), ("sigma", C.c_double), ("p0", C.c_double),
+        ("seed", C.c_uint64), ("uniforms", _dp),
+    ]
+
+
 SIGNATURES = {
     "kg_create": (C.c_int32, [C.POINTER(KGDesc), C.POINTER(_vp)]),
     "kg_destroy": (C.c_int32, [_vp]),
@@ -83,6 +94,9 @@ SIGNATURES = {
     "kg_diag_integrate": (C.c_int32, [_vp, C.c_int32, _dp]),
     "kg_diag_last_error": (C.c_char_p, []),
     "kg_diag_output": (C.c_int32, [_vp, C.c_int32, _dp]),
+    "kg_initial_data": (C.c_int32, [_vp, C.POINTER(KGICDesc), _dp]),
+    "kg_ic_random_field": (C.c_int32, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, _dp, _dp]),
+    "kg_ic_last_error": (C.c_char_p, []),
 }
 
 

@@ -921,6 +921,7 @@ int handle_view(const kg_handle* h, HandleView* v) {
   for (int f = 0; f < kNumFields; ++f) v->u[f] = h->u.f[f];
   v->tab = h->d_tab;
   v->m = h->desc.m;
+  v->metric = h->desc.metric;
   return 0;
 }
 }  // namespace kg

@@ -27,8 +27,9 @@ struct IntArgs {
   int64_t sys_stride;
   int64_t plane;
   int ld, nj, g, Nx, Ny, Nz, nsys;
-  int quantity;  // 0..4 field, 5 energy density times sqrt(gamma)
+  int quantity;  // 0..4 field, 5 density*sqrtg, 6 (density*alpha)*sqrtg, 7 (m*|phi|**2)*sqrtg
   double m2;     // m**2
+  double m;
 };
 
 __device__ __forceinline__ double mag(double re, double im, bool cplx) { return cplx ? hypot(re, im) : fabs(re); }
@@ -71,9 +72,16 @@ __global__ __launch_bounds__(256) void integrate_rows(const IntArgs a, double* _
     const uint32_t t = (uint32_t)(jj * a.ld + ii);
     const int64_t o = (int64_t)kk * a.plane + t;
     const double w = (ir == 0 || ir == a.Nx) ? 0.5 : 1.0;
+    const int64_t n2 = (int64_t)a.nj * a.ld;
     if (a.quantity == 5) {
-      const int64_t n2 = (int64_t)a.nj * a.ld;
       sr += w * (density(a, o, t, cplx) * a.tab[T_SQRTG * n2 + t]);
+    } else if (a.quantity == 6) {
+      // MoL%rho*MoL%alpha*MoL%sqrtg (initial_conditions.f90:86)
+      sr += w * (density(a, o, t, cplx) * a.tab[T_ALPHA * n2 + t] * a.tab[T_SQRTG * n2 + t]);
+    } else if (a.quantity == 7) {
+      // MoL%m*abs(MoL%u(1,:,:,:))**2 *MoL%sqrtg (initial_conditions.f90:142)
+      const double p = mag(a.u[0][o], cplx ? a.u[0][o + a.sys_stride] : 0.0, cplx);
+      sr += w * (a.m * (p * p) * a.tab[T_SQRTG * n2 + t]);
     } else {
       sr += w * a.u[a.quantity][o];
       if (cplx) si += w * a.u[a.quantity][o + a.sys_stride];
@@ -159,17 +167,7 @@ kg_status dfail(kg_status st, const char* msg, hipError_t e = hipSuccess) {
   return st;
 }
 
-}  // namespace
-
-const char* kg_diag_last_error(void) { return g_diag_error.c_str(); }
-
-kg_status kg_diag_integrate(kg_handle* h, int32_t quantity, double out[2]) {
-  if (!h || !out) return dfail(KG_ERR_INVALID, "null argument");
-  if (quantity < 0 || quantity > KG_DIAG_DENSITY) return dfail(KG_ERR_INVALID, "quantity must be 0..4 or KG_DIAG_DENSITY");
-  HandleView v;
-  if (handle_view(h, &v) != 0) return dfail(KG_ERR_INVALID, "bad handle");
-  hipError_t e = hipSetDevice(v.device);
-  if (e != hipSuccess) return dfail(KG_ERR_HIP, "hipSetDevice", e);
+IntArgs make_args(const HandleView& v, int quantity) {
   IntArgs a;
   for (int f = 0; f < 5; ++f) a.u[f] = v.u[f];
   a.tab = v.tab;
@@ -184,10 +182,21 @@ kg_status kg_diag_integrate(kg_handle* h, int32_t quantity, double out[2]) {
   a.nsys = v.nsys;
   a.quantity = quantity;
   a.m2 = v.m * v.m;
+  a.m = v.m;
+  return a;
+}
+
+}  // namespace
+
+const char* kg_diag_last_error(void) { return g_diag_error.c_str(); }
+
+namespace kg {
+hipError_t diag_integrate(const HandleView& v, int quantity, double out[2]) {
+  const IntArgs a = make_args(v, quantity);
   const int nrows = (a.Ny + 1) * (a.Nz + 1);
   double* buf = nullptr;
-  e = hipMalloc((void**)&buf, sizeof(double) * (2 * (size_t)nrows + 2));
-  if (e != hipSuccess) return dfail(KG_ERR_NOMEM, "hipMalloc", e);
+  hipError_t e = hipMalloc((void**)&buf, sizeof(double) * (2 * (size_t)nrows + 2));
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(integrate_rows, dim3(nrows), dim3(256), 0, v.stream, a, buf + 2);
   e = hipGetLastError();
   if (e == hipSuccess) {
@@ -198,6 +207,19 @@ kg_status kg_diag_integrate(kg_handle* h, int32_t quantity, double out[2]) {
   if (e == hipSuccess) e = hipMemcpyAsync(out, buf, 2 * sizeof(double), hipMemcpyDeviceToHost, v.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
   (void)hipFree(buf);
+  return e;
+}
+}  // namespace kg
+
+kg_status kg_diag_integrate(kg_handle* h, int32_t quantity, double out[2]) {
+  if (!h || !out) return dfail(KG_ERR_INVALID, "null argument");
+  if (quantity < 0 || quantity > KG_DIAG_DENSITY) return dfail(KG_ERR_INVALID, "quantity must be 0..4 or KG_DIAG_DENSITY");
+  HandleView v;
+  if (handle_view(h, &v) != 0) return dfail(KG_ERR_INVALID, "bad handle");
+  hipError_t e = hipSetDevice(v.device);
+  if (e != hipSuccess) return dfail(KG_ERR_HIP, "hipSetDevice", e);
+  e = diag_integrate(v, quantity, out);
+  if (e == hipErrorOutOfMemory) return dfail(KG_ERR_NOMEM, "hipMalloc", e);
   if (e != hipSuccess) return dfail(KG_ERR_HIP, "integrate", e);
   return KG_OK;
 }
@@ -209,20 +231,7 @@ kg_status kg_diag_output(kg_handle* h, int32_t quantity, double* out) {
   if (handle_view(h, &v) != 0) return dfail(KG_ERR_INVALID, "bad handle");
   hipError_t e = hipSetDevice(v.device);
   if (e != hipSuccess) return dfail(KG_ERR_HIP, "hipSetDevice", e);
-  IntArgs a;
-  for (int f = 0; f < 5; ++f) a.u[f] = v.u[f];
-  a.tab = v.tab;
-  a.sys_stride = v.G.sys_stride;
-  a.plane = v.G.plane;
-  a.ld = v.G.ld;
-  a.nj = v.G.nj;
-  a.g = v.G.g;
-  a.Nx = v.G.Nx;
-  a.Ny = v.G.Ny;
-  a.Nz = v.G.Nz;
-  a.nsys = v.nsys;
-  a.quantity = quantity;
-  a.m2 = v.m * v.m;
+  const IntArgs a = make_args(v, quantity);
   const int ni_o = a.Nx + a.g + 1, nj_o = a.Ny + 1, nk_o = a.Nz;
   const size_t n = (size_t)ni_o * nj_o * nk_o;
   double* buf = nullptr;

@@ -141,8 +141,14 @@ struct HandleView {
   double* u[5];
   const double* tab;  // kNumTables metric tables [nj][ld]
   double m;           // field mass
+  int metric;         // KG_METRIC_*
 };
 int handle_view(const ::kg_handle* h, HandleView* v);
+
+// Volume integral (integral.f90:646-669) of an integrand of the state, on the
+// view's stream, synchronous.  quantity 0..4: that field; 5: density*sqrtg;
+// 6: (density*alpha)*sqrtg; 7: (m*|phi|**2)*sqrtg.  out = real, imaginary.
+hipError_t diag_integrate(const HandleView& v, int quantity, double out[2]);
 
 // Special functions for the spectral tables (kg_special.cpp, host).
 double ylm_norm_plm(int l, int m, double x);  // N_lm P_l^m(x), Condon-Shortley phase

@@ -1,17 +1,75 @@
-"""Synthetic initial data on the padded grid: Gaussian shells times real Y_lm.
+"""Initial data.
 
-The reference's shell uses exp(-(x - mu)**2/sigma**2) (main/initial_conditions.f90:27,60).
-Here phi = A exp(-(r - r0)**2/w**2) Y(theta, phi) is evaluated at every padded
-point (r = x_i, theta = y_j, phi = z_k of MoL/mesh_refinement_lib.f90:130-132),
-psi_r, psi_theta, psi_phi are its exact partial derivatives and Pi = 0.
-Host-side input generation only; the path itself runs on the GPU.
+- ``initial_conditions``: the reference's ``subroutine initial_conditions``
+  (main/initial_conditions.f90:1-220) for 'spherical_shell', 'random' and
+  'random_shell', built on the GPU (``kg_initial_data``), and
+  ``random_field``, the Cartesian random field of tools/random.f90:17-59
+  (``kg_ic_random_field``).
+- ``shell_state`` and its relatives: synthetic workloads, Gaussian shells
+  times real Y_lm evaluated at every padded point (r = x_i, theta = y_j,
+  phi = z_k of MoL/mesh_refinement_lib.f90:130-132) with exact partial
+  derivatives and Pi = 0; host-side input generation for tests and benches.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
 from scipy.special import lpmv
+
+from . import _capi as capi
+
+IC_TYPES = {"spherical_shell": capi.KG_IC_SPHERICAL_SHELL, "random": capi.KG_IC_RANDOM,
+            "random_shell": capi.KG_IC_RANDOM_SHELL}
+
+
+def _uniform_ptr(uniforms, n):
+    if uniforms is None:
+        return None, None
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    if u.size != n ** 3:
+        raise ValueError("uniforms must hold ncart**3 values")
+    return u, u.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def initial_conditions(lm, initial_conditions_type, mu=0.0, sigma=1.0, p0=1.0, ncart=256, seed=0, uniforms=None):
+    """Fill lm's state with the reference's initial data; returns Amp.
+
+    mu, sigma: the shell (global_numbers); p0: momentum width of the random
+    field (the reference passes 1); ncart: Cartesian FFT points per axis (the
+    reference fixes 256); uniforms: optional random phases U [k][j][i] in
+    [0, 1), otherwise drawn from `seed`.  The random types need a complex lm.
+    """
+    if initial_conditions_type not in IC_TYPES:
+        raise capi.KGError(capi.KG_ERR_UNSUPPORTED, "Intial condition is not implemented yet")
+    d = capi.KGICDesc()
+    d.type = IC_TYPES[initial_conditions_type]
+    d.ncart = ncart
+    d.mu, d.sigma, d.p0 = mu, sigma, p0
+    d.seed = seed & 0xFFFFFFFFFFFFFFFF
+    keep, ptr = _uniform_ptr(uniforms, ncart)
+    d.uniforms = ptr
+    amp = C.c_double()
+    lib = capi.load()
+    st = lib.kg_initial_data(lm._h, C.byref(d), C.byref(amp))
+    if st != capi.KG_OK:
+        raise capi.KGError(st, lib.kg_ic_last_error().decode())
+    del keep
+    return amp.value
+
+
+def random_field(ncart, r_max, p0=1.0, seed=0, uniforms=None, device=0):
+    """psi_x = ifft_3D(psi_p) of random.f90:17-59 on the GPU: complex [k][j][i]."""
+    keep, ptr = _uniform_ptr(uniforms, ncart)
+    out = np.empty((ncart, ncart, ncart), dtype=np.complex128)
+    lib = capi.load()
+    st = lib.kg_ic_random_field(device, ncart, r_max, p0, seed & 0xFFFFFFFFFFFFFFFF, ptr,
+                                out.ctypes.data_as(C.POINTER(C.c_double)))
+    if st != capi.KG_OK:
+        raise capi.KGError(st, lib.kg_ic_last_error().decode())
+    del keep
+    return out
 
 
 def grid_coords(Nx, Ny, Nz, g, xmin, xmax):

@@ -391,3 +391,64 @@ def trapezium_2d_complex(f, dx, dy):
     o_r, o_i = C.c_double(), C.c_double()
     _diag().kgd_trapezium_2d_complex(f.shape[0], f.shape[1], dx, dy, _ptr(fr), _ptr(fi), C.byref(o_r), C.byref(o_i))
     return complex(o_r.value, o_i.value)
+
+
+# --------------------------------------------------------------------------
+# Initial data (kg_ic_oracle.c)
+# --------------------------------------------------------------------------
+IC_TYPES = {"spherical_shell": 0, "random": 1, "random_shell": 2}
+
+
+def _ic():
+    L = lib()
+    if not getattr(L, "_ic_ready", False):
+        L.kgi_uniforms.argtypes = [C.c_uint64, C.c_int64, _dp]
+        L.kgi_random_field.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.kgi_sample.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        L.kgi_initial.restype = C.c_double
+        L.kgi_initial.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp,
+                                  _dp, _dp]
+        L._ic_ready = True
+    return L
+
+
+def uniforms(seed, n):
+    """The n**3 phase numbers U [k][j][i] drawn from `seed`."""
+    out = np.zeros((n, n, n))
+    _ic().kgi_uniforms(seed, n ** 3, _ptr(out))
+    return out
+
+
+def random_field(n, r_max, p0, U):
+    """psi_x = ifft_3D(psi_p) (random.f90:17-59): complex [k][j][i]."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    assert U.shape == (n, n, n)
+    re, im = np.zeros((n, n, n)), np.zeros((n, n, n))
+    _ic().kgi_random_field(n, r_max, p0, _ptr(U), _ptr(re), _ptr(im))
+    return re + 1j * im
+
+
+def sample(sim, field, r_max):
+    """convert_to_spherical (random.f90:128-156) of a complex [k][j][i] field: complex [nk][nj][ni]."""
+    n = field.shape[0]
+    fr, fi = np.ascontiguousarray(field.real), np.ascontiguousarray(field.imag)
+    pr, pi = np.zeros((sim.nk, sim.nj, sim.ni)), np.zeros((sim.nk, sim.nj, sim.ni))
+    _ic().kgi_sample(sim._h, n, r_max, _ptr(fr), _ptr(fi), _ptr(pr), _ptr(pi))
+    return pr + 1j * pi
+
+
+def initial_conditions(sim, kind, mu=0.0, sigma=1.0, field=None, r_max=0.0):
+    """initial_conditions (initial_conditions.f90:23-188): (state complex [5][nk][nj][ni], Amp).
+
+    For the random kinds `field` is psi_x of random_field and r_max the value
+    it was built with."""
+    t = IC_TYPES[kind]
+    ure = np.zeros(sim.shape)
+    uim = np.zeros(sim.shape)
+    if t == 0:
+        amp = _ic().kgi_initial(sim._h, 0, mu, sigma, 0, 0.0, None, None, _ptr(ure), _ptr(uim))
+    else:
+        n = field.shape[0]
+        fr, fi = np.ascontiguousarray(field.real), np.ascontiguousarray(field.imag)
+        amp = _ic().kgi_initial(sim._h, t, mu, sigma, n, r_max, _ptr(fr), _ptr(fi), _ptr(ure), _ptr(uim))
+    return ure + 1j * uim, amp

@@ -48,6 +48,24 @@ def test_spec_desc_struct_matches_header_layout():
     assert _capi.KGSpecDesc.R_max.offset == 16
 
 
+def test_ic_desc_struct_matches_header_layout():
+    from kg_amd import _capi
+    # 2 int32, 3 doubles, uint64, pointer
+    assert C.sizeof(_capi.KGICDesc) == 8 + 3 * 8 + 8 + 8
+    assert _capi.KGICDesc.mu.offset == 8 and _capi.KGICDesc.uniforms.offset == 40
+
+
+def test_random_field_rejects_bad_sizes_before_touching_a_device():
+    from kg_amd import _capi
+    lib = _capi.load()
+    out = (C.c_double * 2)()
+    for n in (0, 3, 6, 2048):
+        assert lib.kg_ic_random_field(0, n, 5.0, 1.0, 0, None, out) == _capi.KG_ERR_INVALID
+    assert b"power of two" in lib.kg_ic_last_error()
+    assert lib.kg_ic_random_field(0, 16, -1.0, 1.0, 0, None, out) == _capi.KG_ERR_INVALID
+    assert lib.kg_ic_random_field(0, 16, 5.0, 1.0, 0, None, None) == _capi.KG_ERR_INVALID
+
+
 def test_create_without_gpu_fails_loudly():
     from conftest import gpu_available
     if gpu_available():
