@@ -16,9 +16,10 @@ Beside it, on the same GPU: the uneven batched workload (256 evolutions, one
 launch per RK4 stage, vs the same grids one at a time) with its
 spherical-Bessel projection (l <= 8, 512 k, split BF16 and FP64), the serving
 loop (Poisson requests, continuous batching, p50/p99 time-to-result), and
-the CPU oracle on a bounded sample.
+the reference's random_shell initial data built on the GPU for the full grid,
+and the CPU oracle on a bounded sample.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batched N] [--serve N]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batched N] [--serve N] [--ic N] [--ic-only]
 """
 from __future__ import annotations
 
@@ -152,6 +153,27 @@ def spectral_projection(members, batched_step_s):
             "fraction_of_50_steps": times["split_bf16"] / (50.0 * batched_step_s)}
 
 
+def initial_data(ncart, reps=3):
+    """The reference's 'random_shell' initial data (initial_conditions.f90:146-188)
+    built on the GPU for the full grid as a complex state: ncart^3 Cartesian random
+    field, its FFT, trilinear sampling, envelope, stencils, fills, normalisation."""
+    import kg_amd
+    from kg_amd.initial_data import initial_conditions
+
+    lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
+                            CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"],
+                            is_complex=True)
+    ts = []
+    for rep in range(reps + 1):
+        t0 = time.perf_counter()
+        amp = initial_conditions(lm, "random_shell", 30.0, 3.0, ncart=ncart, seed=rep)
+        ts.append(time.perf_counter() - t0)
+    lm.close()
+    ts = sorted(ts[1:])
+    return {"kind": "random_shell", "ncart": ncart, "cartesian_bytes": 16 * ncart ** 3, "ms": ts[len(ts) // 2] * 1e3,
+            "amp": amp, "state": "complex, 5 fields on the full grid"}
+
+
 def serving(nreq, tuning):
     """Serving loop: nreq requests of the batched workload arriving as a Poisson
     stream (50/s), steps ~U{500..5000}, admitted against the free HBM and run by
@@ -192,6 +214,9 @@ def main():
     ap.add_argument("--batched", type=int, default=256,
                     help="members of the uneven batched workload to time beside the plain path (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--ic", type=int, default=512,
+                    help="Cartesian points per axis of the random_shell initial data to time (0: skip)")
+    ap.add_argument("--ic-only", action="store_true", help="time only the initial data and exit")
     ap.add_argument("--serve", type=int, default=8,
                     help="requests of the serving loop to run beside the plain path (0: skip)")
     args = ap.parse_args()
@@ -206,6 +231,9 @@ def main():
     from kg_amd.initial_data import random_lm_coeffs, shell_state_device
 
     assert torch.cuda.is_available(), "no GPU visible"
+    if args.ic_only:
+        print(json.dumps({"initial_data": initial_data(args.ic or 512)}))
+        return
     print("bench: building initial data", file=sys.stderr, flush=True)
     lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
                             CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"])
@@ -276,6 +304,9 @@ def main():
                      "bytes_per_launch": per_launch_bytes},
         "fill_ms_per_step": fill_ms / args.steps,
     }
+    if args.ic > 0:
+        print("bench: initial data, ncart %d" % args.ic, file=sys.stderr, flush=True)
+        out["initial_data"] = initial_data(args.ic)
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
         out["batched"] = batched_vs_plain(args.batched, args.batched_steps, best)

@@ -4,14 +4,18 @@
 ! usage: kg_evolve Nx Ny Nz xmin xmax m scheme nsteps in.bin out.bin
 !   scheme: pirk2 | pirk3 | rk3 | rk4
 !   in.bin/out.bin: u(5, -g:Nx+g, -g:Ny+g, 0:Nz) complex(8), raw stream
+!   in.bin may instead be ic=<type>,<mu>,<sigma>,<ncart>,<seed>: the initial
+!   data of main/initial_conditions.f90 built by the engine
 program kg_evolve
   use kg_hip_mod
   implicit none
   type(kg_lines_method) :: MoL
   complex(kind=8), allocatable :: u(:,:,:,:)
   character(len=256) :: arg, fin, fout, scheme
-  integer :: Nx, Ny, Nz, nsteps, l, g, isch
-  real(kind=8) :: xmin, xmax, m
+  integer :: Nx, Ny, Nz, nsteps, l, g, isch, ncart
+  integer(kind=8) :: seed
+  real(kind=8) :: xmin, xmax, m, mu, sigma, amp
+  character(len=64) :: ictype
 
   g = 2
   call get_command_argument(1, arg); read(arg, *) Nx
@@ -35,10 +39,16 @@ program kg_evolve
   MoL = kg_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
                                     'outflow', 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true.)
   allocate(u(5, -g:Nx+g, -g:Ny+g, 0:Nz))
-  open(unit=10, file=trim(fin), access='stream', form='unformatted', status='old')
-  read(10) u
-  close(10)
-  call MoL%upload(u)
+  if (fin(1:3) == 'ic=') then
+    read(fin(4:), *) ictype, mu, sigma, ncart, seed
+    call MoL%initial_conditions(trim(ictype), mu, sigma, ncart, seed, amp)
+    print '(A,A,A,ES24.16)', 'initial data ', trim(ictype), ' Amp ', amp
+  else
+    open(unit=10, file=trim(fin), access='stream', form='unformatted', status='old')
+    read(10) u
+    close(10)
+    call MoL%upload(u)
+  end if
 
   do l = 1, nsteps
     select case (isch)

@@ -29,6 +29,15 @@ module kg_hip_mod
     real(c_double) :: xmin, xmax, tmin, CFL, dt, a, M_bh, m
   end type kg_desc
 
+  ! initial data (include/kg.h: struct kg_ic_desc); seed holds the bits of a uint64_t
+  integer(c_int32_t), parameter, public :: KG_IC_SPHERICAL_SHELL = 0, KG_IC_RANDOM = 1, KG_IC_RANDOM_SHELL = 2
+  type, bind(C), public :: kg_ic_desc
+    integer(c_int32_t) :: type, ncart
+    real(c_double) :: mu, sigma, p0
+    integer(c_int64_t) :: seed
+    type(c_ptr) :: uniforms
+  end type kg_ic_desc
+
   interface
     function kg_create(desc, h) bind(C, name="kg_create") result(st)
       import :: kg_desc, c_ptr, c_int32_t
@@ -118,8 +127,20 @@ module kg_hip_mod
       type(c_ptr), value :: h
       integer(c_int32_t) :: st
     end function kg_sync
+    function kg_initial_data(h, desc, amp) bind(C, name="kg_initial_data") result(st)
+      import :: kg_ic_desc, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: h
+      type(kg_ic_desc), intent(in) :: desc
+      real(c_double), intent(out) :: amp
+      integer(c_int32_t) :: st
+    end function kg_initial_data
+    function kg_ic_last_error() bind(C, name="kg_ic_last_error") result(msg)
+      import :: c_ptr
+      type(c_ptr) :: msg
+    end function kg_ic_last_error
   end interface
 
+  public :: kg_initial_data, kg_ic_last_error
   public :: kg_create, kg_destroy, kg_last_error, kg_get_time, kg_set_time, kg_upload_state, &
             kg_download_state, kg_upload_aux, kg_download_aux, kg_fill_ghosts, kg_pirk2, kg_pirk3, &
             kg_rk3, kg_rk4, kg_step, kg_sync
@@ -140,6 +161,7 @@ module kg_hip_mod
     procedure :: rk3 => klm_rk3
     procedure :: rk4 => klm_rk4
     procedure :: step => klm_step
+    procedure :: initial_conditions => klm_initial_conditions
     procedure :: destroy => klm_destroy
   end type kg_lines_method
 
@@ -265,6 +287,43 @@ contains
     call kg_check(kg_step(this%h, int(n, c_int32_t)), this%h)
     call sync_time(this)
   end subroutine klm_step
+
+  ! initial_conditions (main/initial_conditions.f90:1-220); the globals
+  ! initial_conditions_type, mu and sigma become arguments, with the Cartesian
+  ! grid size of the random types (the reference fixes 256) and their seed
+  subroutine klm_initial_conditions(this, initial_conditions_type, mu, sigma, ncart, seed, amp)
+    class(kg_lines_method), intent(inout) :: this
+    character(len=*), intent(in) :: initial_conditions_type
+    real(kind=8), intent(in) :: mu, sigma
+    integer, intent(in) :: ncart
+    integer(kind=8), intent(in) :: seed
+    real(kind=8), intent(out) :: amp
+    type(kg_ic_desc) :: d
+    character(kind=c_char), pointer :: msg(:)
+    integer(c_int32_t) :: st
+    integer :: n
+    select case (trim(initial_conditions_type))
+    case ('spherical_shell'); d%type = KG_IC_SPHERICAL_SHELL
+    case ('random');          d%type = KG_IC_RANDOM
+    case ('random_shell');    d%type = KG_IC_RANDOM_SHELL
+    case default
+      print*, 'Intial condition is not implemented yet'
+      stop
+    end select
+    d%ncart = int(ncart, c_int32_t); d%mu = mu; d%sigma = sigma; d%p0 = 1.0d0
+    d%seed = int(seed, c_int64_t); d%uniforms = c_null_ptr
+    st = kg_initial_data(this%h, d, amp)
+    if (st /= KG_OK) then
+      call c_f_pointer(kg_ic_last_error(), msg, [512])
+      n = 0
+      do while (n < 512)
+        if (msg(n + 1) == c_null_char) exit
+        n = n + 1
+      end do
+      print*, 'kg engine error ', st, ': ', transfer(msg(1:n), repeat(' ', n))
+      stop
+    end if
+  end subroutine klm_initial_conditions
 
   subroutine klm_destroy(this)
     class(kg_lines_method), intent(inout) :: this
