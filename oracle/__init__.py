@@ -53,9 +53,12 @@ def _load():
     lib.kgo_info.argtypes = [C.c_void_p, _i64p, _dp]
     lib.kgo_reference_dt.restype = C.c_double
     lib.kgo_reference_dt.argtypes = [C.c_int] * 4 + [C.c_double] * 3
-    for nm in ("kgo_dx1_raw", "kgo_dx2_raw", "kgo_dx3_raw"):
+    for nm in ("kgo_dx1_raw", "kgo_dx2_raw", "kgo_dx3_raw", "kgo_dx1_4_raw", "kgo_dx2_4_raw", "kgo_dx3_4_raw"):
         getattr(lib, nm).argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp]
     lib.kgo_advec_x_raw.argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp, _dp]
+    lib.kgo_advec_x_4_raw.argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp, _dp]
+    lib.kgo_set_order.argtypes = [C.c_void_p, C.c_int]
+    lib.kgo_set_order.restype = C.c_int
     for nm in ("kgo_rhs_phi", "kgo_rhs_psi1", "kgo_rhs_psi2", "kgo_rhs_psi3", "kgo_rhs_pi", "kgo_rhs_all"):
         getattr(lib, nm).argtypes = [C.c_void_p, _dp, _dp]
     lib.kgo_fill.argtypes = [C.c_void_p, _dp, _dp]
@@ -136,6 +139,11 @@ class Sim:
         p = lib().kgo_table(self._h, {"x": 10, "y": 11, "z": 12}[which])
         return np.ctypeslib.as_array(p, shape=(n,)).copy()
 
+    def set_order(self, order):
+        """Operator set of the RHS: 2 (the reference's live stencils) or 4 (its fd4 set)."""
+        if lib().kgo_set_order(self._h, order) != 0:
+            raise ValueError("order must be 2 or 4")
+
     def step(self, scheme, nsteps=1):
         lib().kgo_step(self._h, SCHEMES[scheme] if isinstance(scheme, str) else scheme, nsteps)
 
@@ -163,9 +171,9 @@ def stencil(name, f, h, beta=None):
     nk, nj, ni = f.shape
     out = np.zeros_like(f)
     L = lib()
-    if name == "advec_x":
+    if name in ("advec_x", "advec_x_4"):
         b = np.ascontiguousarray(beta, dtype=np.float64)
-        L.kgo_advec_x_raw(ni, nj, nk, h, _ptr(f), _ptr(b), _ptr(out))
+        getattr(L, "kgo_%s_raw" % name)(ni, nj, nk, h, _ptr(f), _ptr(b), _ptr(out))
     else:
         getattr(L, "kgo_%s_raw" % name)(ni, nj, nk, h, _ptr(f), _ptr(out))
     return out
@@ -179,13 +187,26 @@ _ref = None
 
 
 def ref_stencil(name, f, h, g, beta=None):
-    """The reference's own stencil (complex(8) in, complex(8) out)."""
+    """The reference's own stencil (complex(8) in and out; real(8) for the fd4 set, names *_4)."""
     global _ref
     if _ref is None:
         _ref = C.CDLL(_REF)
         for nm in ("kgref_dx1", "kgref_dx2", "kgref_dx3"):
             getattr(_ref, nm).argtypes = [C.c_int] * 4 + [C.c_double, C.c_void_p, C.c_void_p]
         _ref.kgref_advec_x.argtypes = [C.c_int] * 4 + [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        for nm in ("kgref_dx1_4", "kgref_dx2_4", "kgref_dx3_4"):
+            getattr(_ref, nm).argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p, C.c_void_p]
+        _ref.kgref_advec_x_4.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+    if name.endswith("_4"):  # the fd4 set: real arrays
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        nk, nj, ni = f.shape
+        out = np.zeros_like(f)
+        if name == "advec_x_4":
+            b = np.ascontiguousarray(beta, dtype=np.float64)
+            _ref.kgref_advec_x_4(ni, nj, nk, h, f.ctypes.data, b.ctypes.data, out.ctypes.data)
+        else:
+            getattr(_ref, "kgref_" + name)(ni, nj, nk, h, f.ctypes.data, out.ctypes.data)
+        return out
     f = np.ascontiguousarray(f, dtype=np.complex128)
     nk, nj, ni = f.shape
     out = np.zeros_like(f)

@@ -153,6 +153,8 @@ static void inverse_point(const double* gdn, double* gup) {
   GUP(3, 3) = g22 * (g11 * g00 - g01 * g01) * idetg;
 }
 
+static void dx_beta_build(kgo_sim* s);
+
 static void metric_build(kgo_sim* s) {
   const int ni = s->ni, nj = s->nj;
   const int64_t n2 = (int64_t)ni * nj;
@@ -229,17 +231,7 @@ static void metric_build(kgo_sim* s) {
   }
   /* dx_beta = Re dx1(beta(1)) (geometry.f90:344); k-independent, so one
    * plane of the 3-D stencil. */
-  {
-    double res = 0.50 / s->dx;
-    for (int jj = 0; jj < nj; ++jj) {
-      const double* f = s->beta_r + IDX2(s, 0, jj);
-      double* df = s->dx_beta + IDX2(s, 0, jj);
-      int lo = 0, hi = ni - 1;
-      df[lo] = (-3.0 * f[lo] + 4.0 * f[lo + 1] - f[lo + 2]) * res;
-      for (int ii = lo + 1; ii <= hi - 1; ++ii) df[ii] = (-f[ii - 1] + f[ii + 1]) * res;
-      df[hi] = (+3.0 * f[hi] - 4.0 * f[hi - 1] + f[hi - 2]) * res;
-    }
-  }
+  dx_beta_build(s);
 }
 
 /* --------------------------------------------------------------------------
@@ -321,6 +313,157 @@ void kgo_advec_x_raw(int ni, int nj, int nk, double h, const double* f, const do
     }
 }
 
+/* --------------------------------------------------------------------------
+ * The fd4 set (finite_differences.f90, real arrays with lower bound 0; N is
+ * the last index along the differentiated axis).  Each row formula is
+ * evaluated in the reference's order; the whole-array statements run one
+ * after another, so a later row wins where rows coincide (not for N >= 4).
+ * -------------------------------------------------------------------------- */
+#define FD4_ROWS(D, A, N, RES)                                                                          \
+  do {                                                                                                  \
+    D(0) = (-25.0 * A(0) + 48.0 * A(1) - 36.0 * A(2) + 16.0 * A(3) - 3.0 * A(4)) * (RES);              \
+    D(1) = (-3.0 * A(0) - 10.0 * A(1) + 18.0 * A(2) - 6.0 * A(3) + 1.0 * A(4)) * (RES);                \
+    for (int q_ = 2; q_ <= (N)-2; ++q_) D(q_) = (A(q_ - 2) - 8.0 * A(q_ - 1) + 8.0 * A(q_ + 1) - A(q_ + 2)) * (RES); \
+    D((N)-1) = (+3.0 * A(N) + 10.0 * A((N)-1) - 18.0 * A((N)-2) + 6.0 * A((N)-3) - 1.0 * A((N)-4)) * (RES); \
+    D(N) = (+25.0 * A(N) - 48.0 * A((N)-1) + 36.0 * A((N)-2) - 16.0 * A((N)-3) + 3.0 * A((N)-4)) * (RES); \
+  } while (0)
+
+void kgo_dx1_4_raw(int ni, int nj, int nk, double h, const double* f, double* df) {
+  /* first_derivative_x_4 (:124-151): res = 1/(12h), N = size(f,1)-1 */
+  const double res = 1.0 / (12.0 * h);
+  const int N = ni - 1;
+  for (int kk = 0; kk < nk; ++kk)
+    for (int jj = 0; jj < nj; ++jj) {
+      const double* a = f + ((int64_t)kk * nj + jj) * ni;
+      double* d = df + ((int64_t)kk * nj + jj) * ni;
+#define A_(q) a[q]
+#define D_(q) d[q]
+      FD4_ROWS(D_, A_, N, res);
+#undef A_
+#undef D_
+    }
+}
+
+void kgo_dx2_4_raw(int ni, int nj, int nk, double h, const double* f, double* df) {
+  /* first_derivative_y_4 (:241-268): N = size(f,2)-1; no pole rows */
+  const double res = 1.0 / (12.0 * h);
+  const int N = nj - 1;
+  for (int kk = 0; kk < nk; ++kk)
+    for (int ii = 0; ii < ni; ++ii) {
+      const double* a = f + (int64_t)kk * nj * ni + ii;
+      double* d = df + (int64_t)kk * nj * ni + ii;
+#define A_(q) a[(int64_t)(q) * ni]
+#define D_(q) d[(int64_t)(q) * ni]
+      FD4_ROWS(D_, A_, N, res);
+#undef A_
+#undef D_
+    }
+}
+
+void kgo_dx3_4_raw(int ni, int nj, int nk, double h, const double* f, double* df) {
+  /* first_derivative_z_4 (:340-367): N = size(f,3)-1; one-sided at both ends */
+  const double res = 1.0 / (12.0 * h);
+  const int N = nk - 1;
+  const int64_t pl = (int64_t)ni * nj;
+  for (int64_t p = 0; p < pl; ++p) {
+    const double* a = f + p;
+    double* d = df + p;
+#define A_(q) a[(int64_t)(q) * pl]
+#define D_(q) d[(int64_t)(q) * pl]
+    FD4_ROWS(D_, A_, N, res);
+#undef A_
+#undef D_
+  }
+}
+
+void kgo_advec_x_4_raw(int ni, int nj, int nk, double h, const double* f, const double* beta, double* df) {
+  /* advec_x_4 (:922-957): res = 1/dx, every row (...)*res/12; rows 0, 1, 2,
+   * N-2, N-1, N fixed; 3..N-3 upwinded by the sign of beta (beta < 0 first) */
+  const double res = 1.0 / h;
+  const int N = ni - 1;
+  for (int kk = 0; kk < nk; ++kk)
+    for (int jj = 0; jj < nj; ++jj) {
+      const int64_t o = ((int64_t)kk * nj + jj) * ni;
+      const double* a = f + o;
+      const double* b = beta + o;
+      double* d = df + o;
+      d[0] = (-25.0 * a[0] + 48.0 * a[1] - 36.0 * a[2] + 16.0 * a[3] - 3.0 * a[4]) * res / 12.0;
+      d[1] = (-3.0 * a[0] - 10.0 * a[1] + 18.0 * a[2] - 6.0 * a[3] + a[4]) * res / 12.0;
+      d[2] = (a[0] - 8.0 * a[1] + 8.0 * a[3] - a[4]) * res / 12.0;
+      for (int i = 3; i <= N - 3; ++i) {
+        if (b[i] < 0.0)
+          d[i] = (3.0 * a[i + 1] + 10.0 * a[i] - 18.0 * a[i - 1] + 6.0 * a[i - 2] - a[i - 3]) * res / 12.0;
+        else if (b[i] > 0.0)
+          d[i] = (-3.0 * a[i - 1] - 10.0 * a[i] + 18.0 * a[i + 1] - 6.0 * a[i + 2] + a[i + 3]) * res / 12.0;
+        else
+          d[i] = (a[i - 2] - 8.0 * a[i - 1] + 8.0 * a[i + 1] - a[i + 2]) * res / 12.0;
+      }
+      d[N - 2] = (a[N - 4] - 8.0 * a[N - 3] + 8.0 * a[N - 1] - a[N]) * res / 12.0;
+      d[N - 1] = (3.0 * a[N] + 10.0 * a[N - 1] - 18.0 * a[N - 2] + 6.0 * a[N - 3] - a[N - 4]) * res / 12.0;
+      d[N] = (25.0 * a[N] - 48.0 * a[N - 1] + 36.0 * a[N - 2] - 16.0 * a[N - 3] + 3.0 * a[N - 4]) * res / 12.0;
+    }
+}
+
+/* d_r beta^r of the metric with the operator set's D_r (geometry.f90:344
+ * uses dx1; the fd4 set takes x_4, so the coefficient is as accurate as the
+ * derivatives it multiplies) */
+static void dx_beta_build(kgo_sim* s) {
+  const int ni = s->ni, nj = s->nj;
+  if (s->order == 4) {
+    kgo_dx1_4_raw(ni, nj, 1, s->dx, s->beta_r, s->dx_beta);
+    return;
+  }
+  double res = 0.50 / s->dx;
+  for (int jj = 0; jj < nj; ++jj) {
+    const double* f = s->beta_r + IDX2(s, 0, jj);
+    double* df = s->dx_beta + IDX2(s, 0, jj);
+    int lo = 0, hi = ni - 1;
+    df[lo] = (-3.0 * f[lo] + 4.0 * f[lo + 1] - f[lo + 2]) * res;
+    for (int ii = lo + 1; ii <= hi - 1; ++ii) df[ii] = (-f[ii - 1] + f[ii + 1]) * res;
+    df[hi] = (+3.0 * f[hi] - 4.0 * f[hi - 1] + f[hi - 2]) * res;
+  }
+}
+
+int kgo_set_order(kgo_sim* s, int order) {
+  if (order != 2 && order != 4) return -1;
+  s->order = order;
+  dx_beta_build(s);
+  return 0;
+}
+
+/* the RHS's operators under the selected set */
+static void op_dr(const kgo_sim* s, const double* f, double* df) {
+  if (s->order == 4)
+    kgo_dx1_4_raw(s->ni, s->nj, s->nk, s->dx, f, df);
+  else
+    kgo_dx1_raw(s->ni, s->nj, s->nk, s->dx, f, df);
+}
+static void op_dth(const kgo_sim* s, const double* f, double* df) {
+  if (s->order == 4)
+    kgo_dx2_4_raw(s->ni, s->nj, s->nk, s->dy, f, df);
+  else
+    kgo_dx2_raw(s->ni, s->nj, s->nk, s->dy, f, df);
+}
+static void op_dph(const kgo_sim* s, const double* f, double* df) {
+  if (s->order == 4)
+    kgo_dx3_4_raw(s->ni, s->nj, s->nk, s->dz, f, df);
+  else
+    kgo_dx3_raw(s->ni, s->nj, s->nk, s->dz, f, df);
+}
+void kgo_advec_x(const kgo_sim* s, const double* f, const double* beta2d, double* df);
+static void op_adv(const kgo_sim* s, const double* f, double* df) {
+  if (s->order != 4) {
+    kgo_advec_x(s, f, s->beta_r, df);
+    return;
+  }
+  /* advec_x_4 with beta(i,j,k) = beta2d(i,j) */
+  double* b3 = (double*)malloc(sizeof(double) * (size_t)s->npts);
+  const int64_t n2 = (int64_t)s->ni * s->nj;
+  for (int kk = 0; kk < s->nk; ++kk) memcpy(b3 + kk * n2, s->beta_r, sizeof(double) * (size_t)n2);
+  kgo_advec_x_4_raw(s->ni, s->nj, s->nk, s->dx, f, b3, df);
+  free(b3);
+}
+
 void kgo_dx1(const kgo_sim* s, const double* f, double* df) { kgo_dx1_raw(s->ni, s->nj, s->nk, s->dx, f, df); }
 void kgo_dx2(const kgo_sim* s, const double* f, double* df) { kgo_dx2_raw(s->ni, s->nj, s->nk, s->dy, f, df); }
 void kgo_dx3(const kgo_sim* s, const double* f, double* df) { kgo_dx3_raw(s->ni, s->nj, s->nk, s->dz, f, df); }
@@ -378,8 +521,8 @@ void kgo_rhs_psi1(kgo_sim* s, double* rhs, const double* vars) {
         int64_t p = IDX(s, ii, jj, kk), p2 = IDX2(s, ii, jj);
         t0[p] = T2(s->alpha) * v5[p];
       }
-  kgo_dx1(s, t0, t1);
-  kgo_advec_x(s, v2, s->beta_r, t2);
+  op_dr(s, t0, t1);
+  op_adv(s, v2, t2);
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
@@ -402,13 +545,13 @@ static void flux_F(kgo_sim* s, const double* vars, double* F) {
 void kgo_rhs_psi2(kgo_sim* s, double* rhs, const double* vars) {
   /* :700  rhs = dx2(alpha*vars(5) + beta*vars(2)) */
   flux_F(s, vars, s->tmp[0]);
-  kgo_dx2(s, s->tmp[0], rhs);
+  op_dth(s, s->tmp[0], rhs);
 }
 
 void kgo_rhs_psi3(kgo_sim* s, double* rhs, const double* vars) {
   /* :711  rhs = dx3(alpha*vars(5) + beta*vars(2)) */
   flux_F(s, vars, s->tmp[0]);
-  kgo_dx3(s, s->tmp[0], rhs);
+  op_dph(s, s->tmp[0], rhs);
 }
 
 void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars) {
@@ -435,10 +578,10 @@ void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars) {
         ft[p] = al * sg * T2(s->g_tt) * v3[p];
         ps[p] = v5[p] * sg;
       }
-  kgo_dx1(s, fr, d1);
-  kgo_dx2(s, ft, d2);
-  kgo_dx3(s, fp, d3);
-  kgo_advec_x(s, ps, s->beta_r, ad);
+  op_dr(s, fr, d1);
+  op_dth(s, ft, d2);
+  op_dph(s, fp, d3);
+  op_adv(s, ps, ad);
   const double m2 = s->m * s->m;
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
@@ -698,6 +841,7 @@ kgo_sim* kgo_create(int Nx, int Ny, int Nz, int g, double xmin, double xmax, int
   s->M_bh = M_bh;
   s->m = m;
   s->bc_rmax = bc_rmax;
+  s->order = 2;
   s->CFL = CFL;
   s->x = alloc0(s->ni);
   s->y = alloc0(s->nj);

@@ -62,6 +62,7 @@ typedef struct kgo_sim {
   double *g_dn, *g_up;
   /* time stepping */
   int bc_rmax;
+  int order;           /* operator set: 2 = the reference's live stencils, 4 = its fd4 set */
   double t, dt, CFL;
   /* state: 5 fields each, npts doubles per field */
   double *u, *u_p, *u_stage, *u_stage_2, *rhs_u, *k1, *k2, *k3, *k4;
@@ -94,6 +95,17 @@ void kgo_dx1_raw(int ni, int nj, int nk, double h, const double* f, double* df);
 void kgo_dx2_raw(int ni, int nj, int nk, double h, const double* f, double* df);
 void kgo_dx3_raw(int ni, int nj, int nk, double h, const double* f, double* df);
 void kgo_advec_x_raw(int ni, int nj, int nk, double h, const double* f, const double* beta3d, double* df);
+
+/* The 4th-order set of finite_differences.f90 (x_4 :124-151, y_4 :241-268,
+ * z_4 :340-367, advec_x_4 :922-957) on dense real arrays. */
+void kgo_dx1_4_raw(int ni, int nj, int nk, double h, const double* f, double* df);
+void kgo_dx2_4_raw(int ni, int nj, int nk, double h, const double* f, double* df);
+void kgo_dx3_4_raw(int ni, int nj, int nk, double h, const double* f, double* df);
+void kgo_advec_x_4_raw(int ni, int nj, int nk, double h, const double* f, const double* beta3d, double* df);
+/* Selects the operator set of the RHS (2 or 4; default 2).  With 4 every
+ * first derivative of lines_method_lib.f90:671-753 uses the fd4 set, and so
+ * does the metric's d_r beta^r.  Returns 0, or -1 for another order. */
+int kgo_set_order(kgo_sim* s, int order);
 
 /* RHS pieces (lines_method_lib.f90:671-753). vars = 5-field state. */
 void kgo_rhs_phi(kgo_sim* s, double* rhs, const double* vars);

@@ -53,3 +53,18 @@ def test_reference_quirk_rows_are_what_the_oracle_states():
     d3 = oracle.ref_stencil("dx3", tag.astype(complex), h, 2).real
     assert np.all(d3[Nz] == tag[Nz - 1] - tag[2])
     assert np.all(d3[0] == tag[1] - tag[Nz - 2])
+
+
+@pytest.mark.parametrize("shape", [(13, 9, 10), (21, 13, 18), (9, 7, 5), (70, 38, 66)])
+@pytest.mark.parametrize("name", ["dx1_4", "dx2_4", "dx3_4", "advec_x_4"])
+def test_fd4_stencil_bit_exact(shape, name):
+    """The 4th-order set (finite_differences.f90:124-151, 241-268, 340-367,
+    922-957) against the reference's compiled functions, every row."""
+    ni, nj, nk = shape
+    rng = np.random.default_rng(zlib.crc32(repr((shape, name, 4)).encode()))
+    f = rng.standard_normal((nk, nj, ni))
+    beta = rng.choice([-0.7, 0.0, 1.3], size=(nk, nj, ni))
+    h = float(rng.uniform(0.05, 0.5))
+    ref = oracle.ref_stencil(name, f, h, 2, beta=beta)
+    got = oracle.stencil(name, f, h, beta=beta)
+    assert np.array_equal(ref, got)
