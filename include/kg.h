@@ -62,6 +62,14 @@ enum {
                             real, imaginary (imaginary only when is_complex) */
 };
 
+/* Operator set of the right-hand side.  KG_OPS_REF2: the reference's live
+ * 2nd-order stencils (finite_differences.f90:17-44,193-239,310-338,767-801).
+ * KG_OPS_FD4: the 4th-order set the same file defines (x_4 :124-151, y_4
+ * :241-268, z_4 :340-367, advec_x_4 :922-957) in every first derivative of
+ * lines_method_lib.f90:671-753 and in the metric's d_r beta^r; for RK3 and
+ * RK4 (the PIRK schemes and kg_batch_step return KG_ERR_UNSUPPORTED). */
+enum { KG_OPS_REF2 = 0, KG_OPS_FD4 = 1 };
+
 /* Descriptor: the arguments of lines_method_constructor
  * (lines_method_lib.f90:49-51) plus the scheme and the value type. */
 typedef struct kg_desc {
@@ -72,7 +80,7 @@ typedef struct kg_desc {
   int32_t metric;     /* KG_METRIC_* */
   int32_t bc_rmax;    /* KG_BC_* */
   int32_t is_complex; /* 1: complex(8) state (two real systems on the device) */
-  int32_t reserved;
+  int32_t op_set;     /* KG_OPS_REF2 (0) or KG_OPS_FD4 (1) */
   double xmin, xmax;  /* radial domain */
   double tmin;        /* initial time */
   double CFL;         /* used when dt <= 0 */

@@ -16,6 +16,7 @@ KG_PIRK2, KG_PIRK3, KG_RK3, KG_RK4 = range(4)
 KG_METRIC_MINKOWSKI, KG_METRIC_KERR_SCHILD, KG_METRIC_MKS, KG_METRIC_SCHW_EF = range(4)
 KG_BC_OUTFLOW, KG_BC_INFLOW = range(2)
 KG_LAYOUT_FORTRAN, KG_LAYOUT_SOA = range(2)
+KG_OPS_REF2, KG_OPS_FD4 = range(2)
 
 # every function include/kg.h declares, with its ctypes signature
 _dp = C.POINTER(C.c_double)
@@ -27,7 +28,7 @@ class KGDesc(C.Structure):
     _fields_ = [
         ("nvars", C.c_int32), ("g_pts", C.c_int32), ("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
         ("scheme", C.c_int32), ("metric", C.c_int32), ("bc_rmax", C.c_int32), ("is_complex", C.c_int32),
-        ("reserved", C.c_int32),
+        ("op_set", C.c_int32),
         ("xmin", C.c_double), ("xmax", C.c_double), ("tmin", C.c_double), ("CFL", C.c_double),
         ("dt", C.c_double), ("a", C.c_double), ("M_bh", C.c_double), ("m", C.c_double),
     ]

@@ -343,6 +343,8 @@ kg_status ensure_buffers(kg_handle* h, int scheme) {
 
 kg_status do_steps(kg_handle* h, int scheme, int nsteps) {
   if (nsteps < 0) return fail(h, KG_ERR_INVALID, "nsteps < 0");
+  if (h->G.fd4 && (scheme == KG_PIRK2 || scheme == KG_PIRK3))
+    return fail(h, KG_ERR_UNSUPPORTED, "the fd4 operator set runs with rk3 and rk4");
   KG_TRY(ensure_buffers(h, scheme));
   for (int it = 0; it < nsteps; ++it) {
     if (h->prof) {
@@ -453,6 +455,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
     kg_handle* h = hs[q];
     if (!h) return fail(lead, KG_ERR_INVALID, "batch: null handle %d", q);
     if (h->desc.scheme != KG_RK4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d is not RK4", q);
+    if (h->G.fd4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d uses the fd4 operator set", q);
     if (h->G.metric_ef != ef) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed metric forms");
     if (h->device != lead->device) return fail(lead, KG_ERR_INVALID, "batch: handles on different devices");
     for (int p = 0; p < q; ++p)
@@ -601,6 +604,11 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
     return fail(nullptr, KG_ERR_UNSUPPORTED, "This boundary condition type for r_max is not implemented yet");
   if (!(d->xmax > d->xmin)) return fail(nullptr, KG_ERR_INVALID, "xmax must exceed xmin");
   if (!(d->dt > 0.0) && !(d->CFL > 0.0)) return fail(nullptr, KG_ERR_INVALID, "need dt > 0 or CFL > 0");
+  if (d->op_set != KG_OPS_REF2 && d->op_set != KG_OPS_FD4) return fail(nullptr, KG_ERR_INVALID, "bad op_set");
+  if (d->op_set == KG_OPS_FD4 && (d->scheme == KG_PIRK2 || d->scheme == KG_PIRK3))
+    return fail(nullptr, KG_ERR_UNSUPPORTED, "the fd4 operator set runs with rk3 and rk4");
+  if (d->op_set == KG_OPS_FD4 && d->g_pts < 2)
+    return fail(nullptr, KG_ERR_INVALID, "the fd4 operator set needs g_pts >= 2");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, KG_ERR_NO_DEVICE, "no GPU visible");
 
@@ -630,6 +638,11 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
   G.rr = 0.50 / h->hg.dx;
   G.rt = 0.50 / h->hg.dy;
   G.rp = 0.50 / h->hg.dz;
+  G.r4r = 1.0 / (12.0 * h->hg.dx);
+  G.r4t = 1.0 / (12.0 * h->hg.dy);
+  G.r4p = 1.0 / (12.0 * h->hg.dz);
+  G.rar = 1.0 / h->hg.dx;
+  G.fd4 = d->op_set == KG_OPS_FD4;
   G.dt = h->hg.dt;
   G.metric_ef = d->metric == KG_METRIC_SCHW_EF;
   G.bc_inflow = d->bc_rmax == KG_BC_INFLOW;
@@ -642,7 +655,7 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
     return KG_ERR_HIP;
   }
   std::vector<double> tab((size_t)kNumTables * G.nj * G.ld);
-  build_tables(h->hg, d->metric, d->a, d->M_bh, d->m, tab.data());
+  build_tables(h->hg, d->metric, d->a, d->M_bh, d->m, G.fd4, tab.data());
   e = hipMalloc(&h->d_tab, tab.size() * sizeof(double));
   if (e == hipSuccess) {
     h->allocs.push_back(h->d_tab);

@@ -43,9 +43,12 @@ struct Geom {
   int64_t fstride;        // plane*nk rounded up: size of one field of one system
   int64_t sys_stride;     // offset between systems inside a field buffer
   double rr, rt, rp;      // 0.5/dx, 0.5/dy, 0.5/dz
+  double r4r, r4t, r4p;   // fd4 set: 1/(12 dx), 1/(12 dy), 1/(12 dz)
+  double rar;             // fd4 set: 1/dx of advec_x_4
   double dt;
   int metric_ef;          // 1: Schw_EF form of rhs_pi (lines_method_lib.f90:738-743)
   int bc_inflow;          // 1: 'inflow' at r_max
+  int fd4;                // 1: the 4th-order operator set
 };
 
 // Host-side table builder (kg_metric.cpp): fills 10 tables [nj][ld] and the
@@ -56,7 +59,8 @@ struct HostGrid {
 };
 int build_grid(int Nx, int Ny, int Nz, int g, double xmin, double xmax, double CFL, double dt_override,
                HostGrid* out);
-int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m, double* tables /*10*nj*ld*/);
+int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m, int fd4,
+                 double* tables /*kNumTables*nj*ld*/);
 
 // Integrator stage kinds (one kernel launch each).
 enum StageMode : int {

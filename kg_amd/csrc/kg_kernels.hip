@@ -566,6 +566,417 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
   }
 }
 
+// ---------------------------------------------------------------------------
+// The fd4 operator set: the reference's 4th-order stencils
+// (tools/finite_differences.f90: x_4 :124-151, y_4 :241-268, z_4 :340-367,
+// advec_x_4 :922-957) in the same right-hand side, for the integrators that
+// fill the ghosts after every stage (RK3, RK4).  The march is stage_body's
+// with wider reaches: 3 halo columns each side in r (the upwinded A_r reaches
+// i-3 and i+3), 2 halo rows each side in theta, and a five-plane phi queue
+// (the own F and phi-flux at k-2..k+2) fed by loads two planes ahead.  The
+// computed box (i in [1, ni-2], j in [g+1, g+Ny-1], k in [0, Nz-1]) uses the
+// one-sided r rows 1 and N-1 (N = ni-1), A_r's rows 1, 2, N-2 and N-1,
+// central theta rows only, and the one-sided phi rows 0, 1 and Nz-1, whose
+// planes beyond the queue are read from global memory.
+// ---------------------------------------------------------------------------
+constexpr int HR4 = 3;  // r halo of the fd4 set
+constexpr int TJ4 = 8;  // theta rows per workgroup of the fd4 stages
+
+template <int MODE, int EF>
+__device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, const int sysidx) {
+  constexpr int TJ = TJ4;
+  static_assert(Combine<MODE>::boxed && ModeTraits<MODE>::n14 && ModeTraits<MODE>::n5,
+                "the fd4 set runs the stages that are followed by a ghost fill");
+  constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
+
+  constexpr int W = TI + 2 * HR4;
+  __shared__ double sG[TJ][W];        // alpha*Pi
+  __shared__ double sPR[TJ][W];       // psi_r
+  __shared__ double sFR[TJ][W];       // r-flux of rhs_pi
+  __shared__ double sPS[TJ][W];       // Pi*sqrtg
+  __shared__ double sF[TJ + 4][TI];   // alpha*Pi + beta*psi_r
+  __shared__ double sFT[TJ + 4][TI];  // theta-flux of rhs_pi
+
+  const Geom& G = a.G;
+  const int ti = tile % a.nti;
+  int tj, tk;
+  {
+    constexpr int P = 2;
+    const int rest = tile / a.nti;
+    const int full = a.ntj / P;
+    const int tiles_full = full * P * a.ntk;
+    if (rest < tiles_full) {
+      const int g2 = rest / (P * a.ntk), r2 = rest % (P * a.ntk);
+      tj = g2 * P + r2 % P;
+      tk = r2 / P;
+    } else {
+      const int r3 = rest - tiles_full, rem = a.ntj - full * P;
+      tj = full * P + r3 % rem;
+      tk = r3 / rem;
+    }
+  }
+  const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
+  const int tid = row * TI + lane;
+  const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
+  const int ii = i0 + lane, jj = j0 + row;
+  const int ni = G.ni, nj = G.nj, Nz = G.Nz, Ny = G.Ny, g = G.g, ld = G.ld;
+  const int NR = ni - 1;  // last r index of the stencils
+  const uint32_t plane8 = (uint32_t)(G.plane * 8);
+  const uint32_t n2b = (uint32_t)((int64_t)nj * ld * 8);
+  const bool valid = ii < ni && jj < nj;
+  const bool act = valid && ii < a.i_hi && jj < a.j_hi;
+  const int k0 = a.k_lo + tk * a.kc;
+  const int k1 = min(k0 + a.kc, a.k_hi);
+  const int64_t sys = (int64_t)sysidx * G.sys_stride;
+  const uint32_t sysb = (uint32_t)(sys * 8);
+  const double* tab = a.tab;
+  auto T = [&](int which, uint32_t off) { return gld(tab, off + (uint32_t)which * n2b); };
+  auto FP = [&](int r, int f) { return a.rb[r] + (int64_t)(f == 4 ? a.ri4[r] : f) * a.per; };
+  auto OFF = [&](int i, int j, int k) { return (uint32_t)((j * ld + i) * 8) + (uint32_t)k * plane8; };
+  const int dis = Nz / 2;
+
+  // stage input at a padded point, with the ghost fill resolved (as stage_body)
+  auto E = [&](int f, uint32_t o) {
+    if constexpr (VIRT) {
+      const double p = gld(FP(R_IN, f), o + sysb), k = gld(FP(R_KIN, f), o + sysb);
+      if constexpr (MODE == M_RK4_S4)
+        return p + k;
+      else
+        return p + 0.50 * k;
+    } else {
+      return gld(FP(R_IN, f), o + sysb);
+    }
+  };
+  struct Src {
+    uint32_t o;
+    int kind;  // 0: evolved value at o; 1: xL extrapolation from o, o+1, o+2; 2: u_p at o (inflow)
+  };
+  auto remap = [&](int i, int j, int k) -> Src {
+    if constexpr (VIRT) {
+      if (i < 1 || i > ni - 2 || j < g + 1 || j > g + Ny - 1 || k < 1 || k > Nz - 1) {
+        if (k == 0)
+          k = Nz - 1;
+        else if (k == Nz)
+          k = 1;
+        if (j <= g) {
+          j = 2 * g + 1 - j;
+          k = (k + dis) % (Nz - 1);
+        } else if (j >= g + Ny) {
+          j = 2 * Ny + 2 * g - 1 - j;
+          k = (k + dis) % (Nz - 1);
+        }
+        if (i == 0) return Src{OFF(1, j, k), 1};
+        if (i == ni - 1) return G.bc_inflow ? Src{OFF(ni - 1, j, k), 2} : Src{OFF(ni - 2, j, k), 0};
+        return Src{OFF(i, j, k), 0};
+      }
+    }
+    return Src{OFF(i, j, k), 0};
+  };
+  auto Ur = [&](int f, const Src& q) -> double {
+    if constexpr (VIRT) {
+      if (q.kind == 1) return 3.0 * E(f, q.o) - 3.0 * E(f, q.o + 8) + E(f, q.o + 16);
+      if (q.kind == 2) return gld(FP(R_IN, f), q.o + sysb);
+    }
+    return E(f, q.o);
+  };
+  auto U = [&](int f, int i, int j, int k) { return Ur(f, remap(i, j, k)); };
+  struct Raw {
+    double p, k;
+  };
+  const double opaque0 = G.dt * 0.0;
+  auto fetch = [&](int f, const Src& q, Raw& r) {
+    if constexpr (VIRT) {
+      if (q.kind == 0) {
+        r.p = gld(FP(R_IN, f), q.o + sysb);
+        r.k = gld(FP(R_KIN, f), q.o + sysb);
+      } else {
+        r.p = Ur(f, q);
+        r.k = opaque0;
+      }
+    } else {
+      r.p = gld(FP(R_IN, f), q.o + sysb);
+      r.k = opaque0;
+    }
+  };
+  auto val = [&](const Raw& r, bool direct) -> double {
+    if constexpr (VIRT) {
+      const double v = (MODE == M_RK4_S4) ? r.p + r.k : r.p + 0.50 * r.k;
+      return direct ? r.p : v;
+    } else {
+      return r.p;
+    }
+  };
+
+  const uint32_t own = valid ? (uint32_t)((jj * ld + ii) * 8) : 0u;
+  const double al = T(T_ALPHA, own), be = T(T_BETA, own);
+  const double dxb = T(T_DXBETA, own), sg = T(T_SQRTG, own);
+  const double asg = T(T_ASG, own), grr = T(T_GRR, own);
+  const double grp = T(T_GRP, own), asgtt = T(T_ASGTT, own);
+  const double gpp = T(T_GPP, own), am2 = T(T_AM2, own);
+  auto Fof = [&](double Pi, double psr) { return al * Pi + be * psr; };
+  auto fpof = [&](double psr, double psp) {
+    if constexpr (EF)
+      return (al * gpp) * psp;
+    else
+      return asg * (grp * psr + gpp * psp);
+  };
+
+  // halo task: 1 = theta row (j0-2, j0-1, j0+TJ, j0+TJ+1), 2 = r column
+  // (i0-3..i0-1, i0+TI..i0+TI+2), 0 = none
+  int hkind = 0, hi = 0, hj = 0, hrow = 0, hcol = 0;
+  if (tid < 4 * TI) {
+    hkind = 1;
+    const int r4 = tid / TI;
+    hi = i0 + (tid % TI);
+    hrow = r4 < 2 ? r4 : TJ + r4;
+    hj = j0 - 2 + hrow;
+    hcol = tid % TI;
+  } else if (tid < 4 * TI + 6 * TJ) {
+    const int u = tid - 4 * TI;
+    const int cc = u % 6;
+    hkind = 2;
+    hrow = u / 6;
+    hj = j0 + hrow;
+    hi = cc < 3 ? i0 - 3 + cc : i0 + TI + (cc - 3);
+    hcol = hi - i0 + HR4;
+  }
+  if (hi < 0 || hi >= ni || hj < 0 || hj >= nj) hkind = 0;
+  const uint32_t hown = hkind ? (uint32_t)((hj * ld + hi) * 8) : 0u;
+  double h0 = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
+  if (hkind == 1) {
+    h0 = T(T_ALPHA, hown);
+    h1 = T(T_BETA, hown);
+    h2 = T(T_ASGTT, hown);
+  } else if (hkind == 2) {
+    h0 = T(T_ALPHA, hown);
+    h1 = T(T_ASG, hown);
+    h2 = T(T_GRR, hown);
+    h3 = T(T_GRP, hown);
+    h4 = T(T_SQRTG, hown);
+  }
+  auto fetch_halo = [&](int k, Raw* x, bool& direct) {
+    if (hkind == 0) return;
+    const Src q = remap(hi, hj, k);
+    direct = q.kind != 0;
+    fetch(4, q, x[0]);
+    fetch(1, q, x[1]);
+    if (hkind == 1)
+      fetch(2, q, x[2]);
+    else if constexpr (!EF)
+      fetch(3, q, x[2]);
+  };
+
+  // own values at planes kk and kk+1; F and the phi-flux at kk-2..kk+1
+  double Pi = 0, psr = 0, pst = 0, psp = 0, Pi1 = 0, psr1 = 0, pst1 = 0, psp1 = 0;
+  double Fm2 = 0, Fm1 = 0, F0 = 0, Fp1 = 0, fpm2 = 0, fpm1 = 0, fp0 = 0, fpp1 = 0;
+  Src qc{0u, 0}, qc1{0u, 0};
+  Raw hr[3] = {{0, 0}, {0, 0}, {0, 0}};
+  bool hdir = false;
+  if (valid) {
+    for (int d = 2; d >= 1; --d) {
+      if (k0 - d < 0) continue;
+      const Src q = remap(ii, jj, k0 - d);
+      const double mPi = Ur(4, q), mpr = Ur(1, q), mpp = Ur(3, q);
+      if (d == 2) {
+        Fm2 = Fof(mPi, mpr);
+        fpm2 = fpof(mpr, mpp);
+      } else {
+        Fm1 = Fof(mPi, mpr);
+        fpm1 = fpof(mpr, mpp);
+      }
+    }
+    qc = remap(ii, jj, k0);
+    Pi = Ur(4, qc);
+    psr = Ur(1, qc);
+    pst = Ur(2, qc);
+    psp = Ur(3, qc);
+    F0 = Fof(Pi, psr);
+    fp0 = fpof(psr, psp);
+    qc1 = remap(ii, jj, k0 + 1);  // k0 + 1 <= Nz
+    Pi1 = Ur(4, qc1);
+    psr1 = Ur(1, qc1);
+    pst1 = Ur(2, qc1);
+    psp1 = Ur(3, qc1);
+    Fp1 = Fof(Pi1, psr1);
+    fpp1 = fpof(psr1, psp1);
+  }
+  fetch_halo(k0, hr, hdir);
+
+  const double r4r = G.r4r, r4t = G.r4t, r4p = G.r4p, rar = G.rar, dt = G.dt;
+  const int c = lane + HR4;
+
+  for (int kk = k0; kk < k1; ++kk) {
+    const uint32_t sk = (uint32_t)kk * plane8;
+    const double hx0 = val(hr[0], hdir), hx1 = val(hr[1], hdir), hx2 = val(hr[2], hdir);
+    if (valid) {
+      sG[row][c] = al * Pi;
+      sPR[row][c] = psr;
+      sF[row + 2][lane] = F0;
+      if constexpr (EF)
+        sFR[row][c] = asg * grr * psr;
+      else
+        sFR[row][c] = asg * (grr * psr + grp * psp);
+      sPS[row][c] = Pi * sg;
+      sFT[row + 2][lane] = asgtt * pst;
+    }
+    if (hkind == 1) {
+      sF[hrow][hcol] = h0 * hx0 + h1 * hx1;
+      sFT[hrow][hcol] = h2 * hx2;
+    } else if (hkind == 2) {
+      sG[hrow][hcol] = h0 * hx0;
+      sPR[hrow][hcol] = hx1;
+      if constexpr (EF)
+        sFR[hrow][hcol] = h1 * h2 * hx1;
+      else
+        sFR[hrow][hcol] = h1 * (h2 * hx1 + h3 * hx2);
+      sPS[hrow][hcol] = hx0 * h4;
+    }
+    // plane kk+2 of the own point, the halo's next plane and phi at the point
+    Raw nr[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    bool ndir = false;
+    Src qn{0u, 0};
+    if (valid && kk + 2 <= Nz) {
+      qn = remap(ii, jj, kk + 2);
+      ndir = qn.kind != 0;
+      fetch(4, qn, nr[0]);
+      fetch(1, qn, nr[1]);
+      fetch(2, qn, nr[2]);
+      fetch(3, qn, nr[3]);
+    }
+    Raw nhr[3] = {{0, 0}, {0, 0}, {0, 0}};
+    bool nhdir = false;
+    if (kk + 1 < k1) fetch_halo(kk + 1, nhr, nhdir);
+    Raw vr{0, 0};
+    if (act) fetch(0, qc, vr);
+    const uint32_t so = sk + sysb + own;
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+
+    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+    double R[5] = {0, 0, 0, 0, 0};
+    double t1 = 0, t2 = 0, t5 = 0;
+    // x_4 on an LDS r-array: rows 1 and N-1 one-sided, central between
+    auto Dr4 = [&](const double(*A)[W]) {
+      if (ii == NR - 1)
+        return (+3.0 * A[row][c + 1] + 10.0 * A[row][c] - 18.0 * A[row][c - 1] + 6.0 * A[row][c - 2] -
+                1.0 * A[row][c - 3]) * r4r;
+      if (ii == 1)
+        return (-3.0 * A[row][c - 1] - 10.0 * A[row][c] + 18.0 * A[row][c + 1] - 6.0 * A[row][c + 2] +
+                1.0 * A[row][c + 3]) * r4r;
+      return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * r4r;
+    };
+    // advec_x_4 at the point: its fixed rows 1, 2, N-2, N-1, upwinded between
+    auto Ar4 = [&](const double(*A)[W]) {
+      if (ii == NR - 1)
+        return (3.0 * A[row][c + 1] + 10.0 * A[row][c] - 18.0 * A[row][c - 1] + 6.0 * A[row][c - 2] -
+                A[row][c - 3]) * rar / 12.0;
+      if (ii == NR - 2 || ii == 2)
+        return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * rar / 12.0;
+      if (ii == 1)
+        return (-3.0 * A[row][c - 1] - 10.0 * A[row][c] + 18.0 * A[row][c + 1] - 6.0 * A[row][c + 2] +
+                A[row][c + 3]) * rar / 12.0;
+      if (be < 0.0)
+        return (3.0 * A[row][c + 1] + 10.0 * A[row][c] - 18.0 * A[row][c - 1] + 6.0 * A[row][c - 2] -
+                A[row][c - 3]) * rar / 12.0;
+      if (be > 0.0)
+        return (-3.0 * A[row][c - 1] - 10.0 * A[row][c] + 18.0 * A[row][c + 1] - 6.0 * A[row][c + 2] +
+                A[row][c + 3]) * rar / 12.0;
+      return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * rar / 12.0;
+    };
+    // y_4 away from its edge rows (the computed rows are all central)
+    auto Dth4 = [&](const double(*A)[TI]) {
+      return (A[row][lane] - 8.0 * A[row + 1][lane] + 8.0 * A[row + 3][lane] - A[row + 4][lane]) * r4t;
+    };
+    // z_4: one-sided rows 0, 1 and Nz-1; planes beyond the queue from `far`
+    auto Dph4 = [&](double qm2, double qm1, double q0, double qp1, double qp2, auto far) {
+      if (kk == Nz - 1)
+        return (+3.0 * qp1 + 10.0 * q0 - 18.0 * qm1 + 6.0 * qm2 - 1.0 * far(Nz - 4)) * r4p;
+      if (kk == 0) return (-25.0 * q0 + 48.0 * qp1 - 36.0 * qp2 + 16.0 * far(3) - 3.0 * far(4)) * r4p;
+      if (kk == 1) return (-3.0 * qm1 - 10.0 * q0 + 18.0 * qp1 - 6.0 * qp2 + 1.0 * far(4)) * r4p;
+      return (qm2 - 8.0 * qm1 + 8.0 * qp1 - qp2) * r4p;
+    };
+    if (act) {
+      if constexpr (CBASE) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) vb[f] = gld(FP(R_BASE, f), so);
+      }
+      if constexpr (CAUX) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) va[f] = gld(FP(R_AUX, f), so);
+      }
+      if constexpr (MODE == M_RK4_S4) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) va2[f] = gld(FP(R_AUX2, f), so);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) vk[f] = gld(FP(R_KIN, f), so);
+      }
+      R[0] = F0;
+      R[1] = Dr4(sG) + be * Ar4(sPR) + dxb * psr;
+      R[2] = Dth4(sF);
+      t1 = Dr4(sFR) / sg;
+      t2 = Dth4(sFT) / sg;
+      t5 = be * Ar4(sPS) / sg;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const double nPi = val(nr[0], ndir), npsr = val(nr[1], ndir);
+    const double npst = val(nr[2], ndir), npsp = val(nr[3], ndir);
+    const double Fp2 = Fof(nPi, npsr), fpp2 = fpof(npsr, npsp);
+    if (act) {
+      const double vphi = val(vr, qc.kind != 0);
+      R[3] = Dph4(Fm2, Fm1, F0, Fp1, Fp2, [&](int k) { return Fof(U(4, ii, jj, k), U(1, ii, jj, k)); });
+      const double dp = Dph4(fpm2, fpm1, fp0, fpp1, fpp2, [&](int k) { return fpof(U(1, ii, jj, k), U(3, ii, jj, k)); });
+      const double t3 = EF ? dp : dp / sg;
+      const double t4 = Pi * dxb;
+      const double t6 = am2 * vphi;
+      R[4] = t1 + t2 + t3 + t4 + t5 - t6;
+      const double cur[5] = {vphi, psr, pst, psp, Pi};
+      if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, R[f] * dt);
+      } else if constexpr (MODE == M_RK4_S4) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * va2[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
+      } else if constexpr (MODE == M_RK3_S1) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + R[f] * dt);
+      } else if constexpr (MODE == M_RK3_S2) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, 0.750 * vb[f] + 0.250 * (cur[f] + R[f] * dt));
+      } else if constexpr (MODE == M_RK3_S3) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] / 3.0 + 2.0 * (cur[f] + R[f] * dt) / 3.0);
+      }
+    }
+    Fm2 = Fm1;
+    Fm1 = F0;
+    F0 = Fp1;
+    Fp1 = Fp2;
+    fpm2 = fpm1;
+    fpm1 = fp0;
+    fp0 = fpp1;
+    fpp1 = fpp2;
+    __syncthreads();
+    Pi = Pi1;
+    psr = psr1;
+    pst = pst1;
+    psp = psp1;
+    Pi1 = nPi;
+    psr1 = npsr;
+    pst1 = npst;
+    psp1 = npsp;
+    qc = qc1;
+    qc1 = qn;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) hr[x] = nhr[x];
+    hdir = nhdir;
+  }
+}
+
+template <int MODE, int EF>
+__global__ __launch_bounds__(TI* TJ4) void stage4_kernel(const StageArgs a) {
+  stage4_body<MODE, EF>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
+}
+
 // One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The
 // second launch bound is the minimum number of waves per SIMD.  RK4 stages 2
 // and 3 ask for 4 (128 VGPRs; 2 over their natural allocation).  The final
@@ -707,6 +1118,17 @@ __global__ __launch_bounds__(256) void pack_kernel(const Geom G, double* __restr
     src[dense] = dst[c][pitched];
 }
 
+template <int MODE>
+hipError_t launch_fd4(const StageArgs& a, hipStream_t s, int nsys) {
+  dim3 grid((unsigned)(a.nti * a.ntj * a.ntk), (unsigned)nsys, 1);
+  dim3 block(TI, TJ4, 1);
+  if (a.G.metric_ef)
+    hipLaunchKernelGGL((stage4_kernel<MODE, 1>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((stage4_kernel<MODE, 0>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <int MODE, int TJ>
 hipError_t launch_mode_tj(const StageArgs& a, hipStream_t s, int nsys) {
   dim3 grid((unsigned)(a.nti * a.ntj * a.ntk), (unsigned)nsys, 1);
@@ -763,6 +1185,20 @@ hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstar
 hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
   StageArgs a = a0;
   const int nsys = a.nsys;
+  if (a.G.fd4) {
+    a.tj = TJ4;
+    finalize_tiles(a);
+    switch (mode) {
+      case M_RK4_S1: return launch_fd4<M_RK4_S1>(a, s, nsys);
+      case M_RK4_S2: return launch_fd4<M_RK4_S2>(a, s, nsys);
+      case M_RK4_S3: return launch_fd4<M_RK4_S3>(a, s, nsys);
+      case M_RK4_S4: return launch_fd4<M_RK4_S4>(a, s, nsys);
+      case M_RK3_S1: return launch_fd4<M_RK3_S1>(a, s, nsys);
+      case M_RK3_S2: return launch_fd4<M_RK3_S2>(a, s, nsys);
+      case M_RK3_S3: return launch_fd4<M_RK3_S3>(a, s, nsys);
+      default: return hipErrorInvalidValue;  // the PIRK stages are refused at the API
+    }
+  }
   finalize_tiles(a);
   switch (mode) {
     case M_RK4_S1: return launch_mode<M_RK4_S1>(a, s, nsys);

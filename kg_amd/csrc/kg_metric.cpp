@@ -132,7 +132,7 @@ G4 inverse(const G4& d) {
 
 }  // namespace
 
-int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m, double* tab) {
+int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m, int fd4, double* tab) {
   const int ni = G.ni, nj = G.nj, ld = G.ld, g = G.g;
   const size_t n2 = (size_t)nj * ld;
   std::memset(tab, 0, sizeof(double) * kNumTables * n2);
@@ -170,13 +170,25 @@ int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m,
       tab[T_GTT * n2 + p] = gtt;
       beta_row[ii] = beta[1];
     }
-    // dx_beta = Re dx1(beta^r) (geometry.f90:344; stencil finite_differences.f90:36-42)
-    const double res = 0.50 / G.dx;
+    // dx_beta = Re dx1(beta^r) (geometry.f90:344; stencil finite_differences.f90:36-42);
+    // with the fd4 set, x_4 (:124-151), as accurate as the derivatives it multiplies
     double* d = tab + T_DXBETA * n2 + (size_t)jj * ld;
-    const int lo = 0, hi = ni - 1;
-    d[lo] = (-3.0 * beta_row[lo] + 4.0 * beta_row[lo + 1] - beta_row[lo + 2]) * res;
-    for (int ii = lo + 1; ii <= hi - 1; ++ii) d[ii] = (-beta_row[ii - 1] + beta_row[ii + 1]) * res;
-    d[hi] = (+3.0 * beta_row[hi] - 4.0 * beta_row[hi - 1] + beta_row[hi - 2]) * res;
+    const double* f = beta_row.data();
+    if (fd4) {
+      const double res = 1.0 / (12.0 * G.dx);
+      const int N = ni - 1;
+      d[0] = (-25.0 * f[0] + 48.0 * f[1] - 36.0 * f[2] + 16.0 * f[3] - 3.0 * f[4]) * res;
+      d[1] = (-3.0 * f[0] - 10.0 * f[1] + 18.0 * f[2] - 6.0 * f[3] + 1.0 * f[4]) * res;
+      for (int ii = 2; ii <= N - 2; ++ii) d[ii] = (f[ii - 2] - 8.0 * f[ii - 1] + 8.0 * f[ii + 1] - f[ii + 2]) * res;
+      d[N - 1] = (+3.0 * f[N] + 10.0 * f[N - 1] - 18.0 * f[N - 2] + 6.0 * f[N - 3] - 1.0 * f[N - 4]) * res;
+      d[N] = (+25.0 * f[N] - 48.0 * f[N - 1] + 36.0 * f[N - 2] - 16.0 * f[N - 3] + 3.0 * f[N - 4]) * res;
+    } else {
+      const double res = 0.50 / G.dx;
+      const int lo = 0, hi = ni - 1;
+      d[lo] = (-3.0 * f[lo] + 4.0 * f[lo + 1] - f[lo + 2]) * res;
+      for (int ii = lo + 1; ii <= hi - 1; ++ii) d[ii] = (-f[ii - 1] + f[ii + 1]) * res;
+      d[hi] = (+3.0 * f[hi] - 4.0 * f[hi - 1] + f[hi - 2]) * res;
+    }
   }
   return 0;
 }

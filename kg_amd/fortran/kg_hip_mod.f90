@@ -25,7 +25,7 @@ module kg_hip_mod
 
   type, bind(C), public :: kg_desc
     integer(c_int32_t) :: nvars, g_pts, Nx, Ny, Nz
-    integer(c_int32_t) :: scheme, metric, bc_rmax, is_complex, reserved
+    integer(c_int32_t) :: scheme, metric, bc_rmax, is_complex, op_set
     real(c_double) :: xmin, xmax, tmin, CFL, dt, a, M_bh, m
   end type kg_desc
 
@@ -220,7 +220,7 @@ contains
       stop
     end if
     d%is_complex = merge(1, 0, is_complex)
-    d%reserved = 0
+    d%op_set = 0
     d%xmin = xmin; d%xmax = xmax; d%tmin = tmin; d%CFL = CFL; d%dt = 0.0d0
     d%a = a; d%M_bh = M_bh; d%m = m
     call kg_check(kg_create(d, this%h), c_null_ptr)

@@ -22,6 +22,9 @@ _SCHEMES = {"pirk2": capi.KG_PIRK2, "pirk3": capi.KG_PIRK3, "rk3": capi.KG_RK3, 
 _METRICS = {"Minkowski": capi.KG_METRIC_MINKOWSKI, "Kerr_Schild": capi.KG_METRIC_KERR_SCHILD,
             "MKS": capi.KG_METRIC_MKS, "Schw_EF": capi.KG_METRIC_SCHW_EF}
 _BCS = {"outflow": capi.KG_BC_OUTFLOW, "inflow": capi.KG_BC_INFLOW}
+# operator set of the RHS: the reference's live 2nd-order stencils, or the
+# 4th-order set its finite_differences.f90 defines (RK3 and RK4 only)
+_OPS = {"ref2": capi.KG_OPS_REF2, "fd4": capi.KG_OPS_FD4}
 
 
 class LinesMethod:
@@ -34,7 +37,7 @@ class LinesMethod:
 
     def __init__(self, nvars=5, g_pts=2, xmin=1.8, xmax=60.0, Nx=256, Ny=33, Nz=65, tmin=0.0, tmax=None,
                  CFL=0.25, boundary_type_rmax="outflow", metric="Kerr_Schild", a=0.0, M_bh=1.0, m=0.0,
-                 scheme="pirk2", dt=0.0, is_complex=False):
+                 scheme="pirk2", dt=0.0, is_complex=False, operators="ref2"):
         if boundary_type_rmax not in _BCS:
             raise capi.KGError(capi.KG_ERR_UNSUPPORTED,
                                "This boundary condition type for r_max is not implemented yet")
@@ -47,6 +50,9 @@ class LinesMethod:
         d.metric = _METRICS[metric]
         d.bc_rmax = _BCS[boundary_type_rmax]
         d.is_complex = 1 if is_complex else 0
+        if operators not in _OPS:
+            raise capi.KGError(capi.KG_ERR_INVALID, "operators must be 'ref2' or 'fd4'")
+        d.op_set = _OPS[operators]
         d.xmin, d.xmax, d.tmin, d.CFL, d.dt = xmin, xmax, tmin, CFL, dt
         d.a, d.M_bh, d.m = a, M_bh, m
         h = C.c_void_p()
@@ -66,6 +72,7 @@ class LinesMethod:
         self.m = m
         self.a = a
         self.M_bh = M_bh
+        self.operators = operators
 
     # -- lifetime -------------------------------------------------------------
     def close(self):
