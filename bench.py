@@ -3,9 +3,11 @@
 Workload (the full single-GPU configuration): massive (m = 0.1) Klein-Gordon
 field on Schwarzschild in Kerr-Schild coordinates (M = 1), Nr = 2048 on
 r in [1.8, 200], Ntheta = 128, Nphi = 256 (67.1 M points x 5 fields), classic
-RK4 with the reference's 2nd-order stencils, dt from CFL 0.25.  Initial phi
-is a Gaussian shell (r0 = 30, width 3) times sum_{l<=4} c_lm Y_lm with
-c_lm ~ N(0, 1e-3) from a seed; psi = its gradient, Pi = 0.
+RK4 with the 4th-order stencil set of tools/finite_differences.f90 (x_4, y_4,
+z_4, advec_x_4), dt from CFL 0.25.  Initial phi is a Gaussian shell (r0 = 30,
+width 3) times sum_{l<=4} c_lm Y_lm with c_lm ~ N(0, 1e-3) from a seed; psi =
+its gradient, Pi = 0.  The same workload with the reference's live 2nd-order
+stencils is timed beside it (`plain_ref2`; --operators ref2 swaps the two).
 
 value = Nr*Ntheta*Nphi * steps / wall time of the timed steps (state resident
 in HBM).  The timed steps run with a HIP event pair around every kernel on
@@ -19,7 +21,8 @@ loop (Poisson requests, continuous batching, p50/p99 time-to-result), and
 the reference's random_shell initial data built on the GPU for the full grid,
 and the CPU oracle on a bounded sample.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batched N] [--serve N] [--ic N] [--ic-only]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--operators fd4|ref2] [--other-steps K]
+                [--batched N] [--serve N] [--ic N] [--ic-only]
 """
 from __future__ import annotations
 
@@ -39,10 +42,10 @@ STAGES = {"rk4": 4, "rk3": 3, "pirk2": 3, "pirk3": 5}
 CFG = dict(Nx=2048, Ny=129, Nz=257, xmin=1.8, xmax=200.0, m=0.1, M_bh=1.0, a=0.0, CFL=0.25, scheme="rk4")
 
 
-def cpu_baseline(max_seconds=20.0):
+def cpu_baseline(operators, max_seconds=20.0):
     """The oracle (C restatement, 1 thread) on a bounded sample of the workload:
     the same radial/polar grid (Nr = 2048, Ntheta = 128) with 8 azimuthal planes,
-    RK4 steps until ~max_seconds of CPU time."""
+    RK4 steps with the same operator set until ~max_seconds of CPU time."""
     import numpy as np
     import oracle
     from kg_amd.initial_data import shell_state, random_lm_coeffs
@@ -50,6 +53,7 @@ def cpu_baseline(max_seconds=20.0):
     Nx, Ny, Nz = CFG["Nx"], CFG["Ny"], 9
     sim = oracle.Sim(Nx, Ny, Nz, 2, CFG["xmin"], CFG["xmax"], "Kerr_Schild", 0.0, 1.0, CFG["m"], "outflow",
                      CFG["CFL"])
+    sim.set_order(4 if operators == "fd4" else 2)
     sim.u[...] = shell_state(Nx, Ny, Nz, 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0, random_lm_coeffs(20261203))
     steps, t0 = 0, time.perf_counter()
     while True:
@@ -60,15 +64,18 @@ def cpu_baseline(max_seconds=20.0):
             break
     ups = Nx * (Ny - 1) * (Nz - 1) * steps / el
     return {"value": ups, "unit": "grid-point updates/s", "cores": 1, "kind": "port",
-            "sample": "oracle/kg_oracle.c RK4, %d steps on Nr=%d, Ntheta=%d, Nphi=%d (the workload's r-theta grid, "
-                      "8 phi planes), 1 thread, %.1f s" % (steps, Nx, Ny - 1, Nz - 1, el)}
+            "sample": "oracle/kg_oracle.c RK4 (%s operators), %d steps on Nr=%d, Ntheta=%d, Nphi=%d (the workload's "
+                      "r-theta grid, 8 phi planes), 1 thread, %.1f s" % (operators, steps, Nx, Ny - 1, Nz - 1, el)}
 
 
-def measured_traffic(tuning):
+TRAFFIC_PROFILES = {"ref2": "stage_kernel_rk4_nr2048_traffic.json", "fd4": "stage4_kernel_rk4_nr2048_traffic.json"}
+
+
+def measured_traffic(tuning, operators):
     """HBM bytes per stage launch from the committed PMC profile of this workload
     (FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a copy
     of known size); None when the launch shape differs from the profiled one."""
-    path = os.path.join(ROOT, "profiles", "stage_kernel_rk4_nr2048_traffic.json")
+    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILES[operators])
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -203,12 +210,82 @@ def serving(nreq, tuning):
                         "chunks of 50 RK4 steps, final fields + l<=8, 512-k spectrum (split-BF16)"}
 
 
+KERNELS = {"ref2": "stage_kernel (RHS + RK4 stage combine, the reference's 2nd-order stencils)",
+           "fd4": "stage4_kernel (RHS + RK4 stage combine, the 4th-order stencil set)"}
+
+
+def plain(operators, steps, warmup, kc=0, tj=8):
+    """The plain single-grid workload with one operator set: state resident in
+    HBM, launch shape autotuned (or fixed), `warmup` untimed steps, `steps`
+    timed steps with HIP events around every stage kernel."""
+    import numpy as np
+    import torch
+    import kg_amd
+    from kg_amd.initial_data import random_lm_coeffs, shell_state_device
+
+    print("bench: building initial data (%s)" % operators, file=sys.stderr, flush=True)
+    lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
+                            CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"],
+                            operators=operators)
+    u0 = shell_state_device(CFG["Nx"], CFG["Ny"], CFG["Nz"], 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0,
+                            random_lm_coeffs(20261203))
+    torch.cuda.synchronize()
+    lm.copy_state_device(u0.data_ptr(), to_handle=True)
+    del u0
+    torch.cuda.empty_cache()
+    lm.boundary()  # ghost-fill the initial data (the reference's IC fills x, y, z ghosts the same way)
+    if kc > 0:
+        best = (kc, tj)
+        lm.set_tuning(*best)
+    else:
+        from kg_amd.autotune import autotune
+        best, timings = autotune(lm, steps=2)
+        print("bench: autotune %s %s" % (operators, {"%d/%d" % k: round(v, 4) for k, v in timings.items()}),
+              file=sys.stderr, flush=True)
+    print("bench: warmup %d steps" % warmup, file=sys.stderr, flush=True)
+    lm.step(warmup)
+    lm.sync()
+    torch.cuda.synchronize()
+    lm.profile(True)
+    torch.cuda.synchronize()
+    lm.sync()
+    t0 = time.perf_counter()
+    lm.step(steps)
+    lm.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stage_ms, stage_n = lm.profile_read(0)
+    fill_ms, fill_n = lm.profile_read(1)
+    lm.profile(False)
+    u = lm.u_soa()[0]
+    finite = bool(np.isfinite(u).all())
+    lm.close()
+    npts = CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1)
+    elapsed = t1 - t0
+    sch = CFG["scheme"]
+    per_launch_bytes = npts * B_PER_UPDATE[sch] / STAGES[sch]
+    avg_stage_s = stage_ms / 1e3 / max(stage_n, 1)
+    achieved = per_launch_bytes / avg_stage_s / 1e9
+    return {"operators": operators, "value": npts * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+            "steps": steps, "state_finite": finite, "tuning": {"kc": best[0], "tj": best[1]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(best, operators),
+                         "kernel": KERNELS[operators], "avg_kernel_ms": avg_stage_s * 1e3, "launches": stage_n,
+                         "bytes_per_launch": per_launch_bytes},
+            "fill_ms_per_step": fill_ms / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--operators", choices=["fd4", "ref2"], default="fd4",
+                    help="operator set of the timed workload: fd4 (the 4th-order stencils the workload names) or "
+                         "ref2 (the reference's live 2nd-order stencils); the other is timed beside it")
+    ap.add_argument("--other-steps", type=int, default=100,
+                    help="timed steps of the other operator set (0: skip)")
     ap.add_argument("--kc", type=int, default=0, help="fix the stage kernel's phi planes per workgroup (skips autotune)")
     ap.add_argument("--tj", type=int, default=8, help="theta rows per workgroup when --kc is given")
     ap.add_argument("--batched", type=int, default=256,
@@ -225,97 +302,53 @@ def main():
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
-    import kg_amd
-    from kg_amd.initial_data import random_lm_coeffs, shell_state_device
 
     assert torch.cuda.is_available(), "no GPU visible"
     if args.ic_only:
         print(json.dumps({"initial_data": initial_data(args.ic or 512)}))
         return
-    print("bench: building initial data", file=sys.stderr, flush=True)
-    lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
-                            CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"])
-    u0 = shell_state_device(CFG["Nx"], CFG["Ny"], CFG["Nz"], 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0,
-                            random_lm_coeffs(20261203))
-    torch.cuda.synchronize()
-    lm.copy_state_device(u0.data_ptr(), to_handle=True)
-    del u0
-    torch.cuda.empty_cache()
-    lm.boundary()  # ghost-fill the initial data (the reference's IC fills x, y, z ghosts the same way)
-    if args.kc > 0:
-        best = (args.kc, args.tj)
-        lm.set_tuning(*best)
-    else:
-        from kg_amd.autotune import autotune
-        best, timings = autotune(lm, steps=2)
-        print("bench: autotune %s" % {"%d/%d" % k: round(v, 4) for k, v in timings.items()}, file=sys.stderr,
-              flush=True)
-    print("bench: warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
-    lm.step(args.warmup)
-    lm.sync()
-    torch.cuda.synchronize()
-
-    lm.profile(True)
-    torch.cuda.synchronize()
-    lm.sync()
-    t0 = time.perf_counter()
-    lm.step(args.steps)
-    lm.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    stage_ms, stage_n = lm.profile_read(0)
-    fill_ms, fill_n = lm.profile_read(1)
-    lm.profile(False)
-
-    u = lm.u_soa()[0]
-    finite = bool(np.isfinite(u).all())
-    lm.close()
-
-    npts = CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1)
-    elapsed = t1 - t0
-    value = npts * args.steps / elapsed
-    sch = CFG["scheme"]
-    per_launch_bytes = npts * B_PER_UPDATE[sch] / STAGES[sch]
-    avg_stage_s = stage_ms / 1e3 / max(stage_n, 1)
-    achieved = per_launch_bytes / avg_stage_s / 1e9
+    head = plain(args.operators, args.steps, args.warmup, args.kc, args.tj)
+    order = "4th-order" if args.operators == "fd4" else "the reference's 2nd-order"
     out = {
         "metric": "KG grid-point updates/s (batched vs plain, <=1e-12 vs FP64 CPU) + p99 latency",
-        "value": value,
+        "value": head["value"],
         "unit": "grid-point updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Gaussian shell x sum of Y_lm, l<=4)",
-        "config": {"workload": "plain single-grid RK4, Nr=2048 r in [1.8,200], Ntheta=128, Nphi=256, m=0.1, "
-                               "Kerr-Schild M=1, CFL 0.25",
-                   "grid_points": npts, "scheme": "rk4", "fields": 5, "state_finite": finite,
-                   "tuning": {"kc": best[0], "tj": best[1]}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(best),
-                     "kernel": "stage_kernel (RHS + RK4 stage combine)",
-                     "avg_kernel_ms": avg_stage_s * 1e3, "launches": stage_n,
-                     "bytes_per_launch": per_launch_bytes},
-        "fill_ms_per_step": fill_ms / args.steps,
+        "config": {"workload": "plain single-grid RK4 with %s stencils, Nr=2048 r in [1.8,200], Ntheta=128, "
+                               "Nphi=256, m=0.1, Kerr-Schild M=1, CFL 0.25" % order,
+                   "grid_points": CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1), "scheme": "rk4",
+                   "operators": args.operators, "fields": 5, "state_finite": head["state_finite"],
+                   "tuning": head["tuning"]},
+        "roofline": head["roofline"],
+        "fill_ms_per_step": head["fill_ms_per_step"],
     }
+    other = "ref2" if args.operators == "fd4" else "fd4"
+    if args.other_steps > 0:
+        print("bench: the %s operator set beside it" % other, file=sys.stderr, flush=True)
+        out["plain_" + other] = plain(other, args.other_steps, 5)
     if args.ic > 0:
         print("bench: initial data, ncart %d" % args.ic, file=sys.stderr, flush=True)
         out["initial_data"] = initial_data(args.ic)
+    best_ref2 = tuple(out["plain_ref2"]["tuning"].values()) if "plain_ref2" in out else (
+        tuple(head["tuning"].values()) if args.operators == "ref2" else (8, 8))
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
-        out["batched"] = batched_vs_plain(args.batched, args.batched_steps, best)
+        out["batched"] = batched_vs_plain(args.batched, args.batched_steps, best_ref2)
     if args.serve > 0:
         print("bench: serving loop, %d requests" % args.serve, file=sys.stderr, flush=True)
-        out["serving"] = serving(args.serve, best)
+        out["serving"] = serving(args.serve, best_ref2)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(args.operators)
     print(json.dumps(out))
 
 

@@ -18,6 +18,9 @@ CANDIDATES = [(kc, tj) for tj in (4, 8) for kc in (8, 16, 32, 64)]
 
 def autotune(lm, candidates=CANDIDATES, steps=2, verbose=False):
     """Pick (kc, tj) for LinesMethod `lm`; returns (best, {shape: ms per stage kernel})."""
+    if getattr(lm, "operators", "ref2") == "fd4":
+        # the fd4 stages fix their theta rows per workgroup: only kc varies
+        candidates = [(kc, 8) for kc in sorted({c[0] for c in candidates})]
     saved = lm.u_soa()
     t_saved = lm.t
     aux = None

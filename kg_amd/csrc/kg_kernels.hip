@@ -580,7 +580,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
 // planes beyond the queue are read from global memory.
 // ---------------------------------------------------------------------------
 constexpr int HR4 = 3;  // r halo of the fd4 set
-constexpr int TJ4 = 8;  // theta rows per workgroup of the fd4 stages
+// theta rows per workgroup of the fd4 stages (12 rows for the ~160-VGPR
+// stages measured no faster on the full grid)
+constexpr int TJ4 = 8;
 
 template <int MODE, int EF>
 __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, const int sysidx) {
@@ -588,6 +590,9 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
   static_assert(Combine<MODE>::boxed && ModeTraits<MODE>::n14 && ModeTraits<MODE>::n5,
                 "the fd4 set runs the stages that are followed by a ghost fill");
   constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
+  // RK4 stage 4 combines u_p and k3 at the point: the march holds both as the
+  // raw operands of its stage input u_p + k3 (planes kk and kk+1 kept raw)
+  constexpr bool KEEP = MODE == M_RK4_S4;
 
   constexpr int W = TI + 2 * HR4;
   __shared__ double sG[TJ][W];        // alpha*Pi
@@ -772,6 +777,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
   Src qc{0u, 0}, qc1{0u, 0};
   Raw hr[3] = {{0, 0}, {0, 0}, {0, 0}};
   bool hdir = false;
+  Raw cr[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}}, cr1[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // KEEP
   if (valid) {
     for (int d = 2; d >= 1; --d) {
       if (k0 - d < 0) continue;
@@ -786,17 +792,35 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
       }
     }
     qc = remap(ii, jj, k0);
-    Pi = Ur(4, qc);
-    psr = Ur(1, qc);
-    pst = Ur(2, qc);
-    psp = Ur(3, qc);
+    qc1 = remap(ii, jj, k0 + 1);  // k0 + 1 <= Nz
+    if constexpr (KEEP) {
+      const int fl[4] = {4, 1, 2, 3};
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        fetch(fl[x], qc, cr[x]);
+        fetch(fl[x], qc1, cr1[x]);
+      }
+      const bool d0 = qc.kind != 0, d1 = qc1.kind != 0;
+      Pi = val(cr[0], d0);
+      psr = val(cr[1], d0);
+      pst = val(cr[2], d0);
+      psp = val(cr[3], d0);
+      Pi1 = val(cr1[0], d1);
+      psr1 = val(cr1[1], d1);
+      pst1 = val(cr1[2], d1);
+      psp1 = val(cr1[3], d1);
+    } else {
+      Pi = Ur(4, qc);
+      psr = Ur(1, qc);
+      pst = Ur(2, qc);
+      psp = Ur(3, qc);
+      Pi1 = Ur(4, qc1);
+      psr1 = Ur(1, qc1);
+      pst1 = Ur(2, qc1);
+      psp1 = Ur(3, qc1);
+    }
     F0 = Fof(Pi, psr);
     fp0 = fpof(psr, psp);
-    qc1 = remap(ii, jj, k0 + 1);  // k0 + 1 <= Nz
-    Pi1 = Ur(4, qc1);
-    psr1 = Ur(1, qc1);
-    pst1 = Ur(2, qc1);
-    psp1 = Ur(3, qc1);
     Fp1 = Fof(Pi1, psr1);
     fpp1 = fpof(psr1, psp1);
   }
@@ -896,9 +920,18 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
       return (qm2 - 8.0 * qm1 + 8.0 * qp1 - qp2) * r4p;
     };
     if (act) {
+      // KEEP: u_p and k3 at the point are the raw operands of the march
+      // (plane 0 resolves to plane Nz-1, so it reads plane 0 itself)
+      const bool kept = KEEP && kk != 0;
       if constexpr (CBASE) {
+        if (kept) {
+          vb[0] = vr.p;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) vb[f] = gld(FP(R_BASE, f), so);
+          for (int x = 0; x < 4; ++x) vb[x == 0 ? 4 : x] = cr[x].p;
+        } else {
+#pragma unroll
+          for (int f = 0; f < 5; ++f) vb[f] = gld(FP(R_BASE, f), so);
+        }
       }
       if constexpr (CAUX) {
 #pragma unroll
@@ -907,8 +940,14 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
       if constexpr (MODE == M_RK4_S4) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) va2[f] = gld(FP(R_AUX2, f), so);
+        if (kept) {
+          vk[0] = vr.k;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) vk[f] = gld(FP(R_KIN, f), so);
+          for (int x = 0; x < 4; ++x) vk[x == 0 ? 4 : x] = cr[x].k;
+        } else {
+#pragma unroll
+          for (int f = 0; f < 5; ++f) vk[f] = gld(FP(R_KIN, f), so);
+        }
       }
       R[0] = F0;
       R[1] = Dr4(sG) + be * Ar4(sPR) + dxb * psr;
@@ -966,6 +1005,13 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
     psp1 = npsp;
     qc = qc1;
     qc1 = qn;
+    if constexpr (KEEP) {
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        cr[x] = cr1[x];
+        cr1[x] = nr[x];
+      }
+    }
 #pragma unroll
     for (int x = 0; x < 3; ++x) hr[x] = nhr[x];
     hdir = nhdir;
@@ -1119,7 +1165,10 @@ __global__ __launch_bounds__(256) void pack_kernel(const Geom G, double* __restr
 }
 
 template <int MODE>
-hipError_t launch_fd4(const StageArgs& a, hipStream_t s, int nsys) {
+hipError_t launch_fd4(const StageArgs& a0, hipStream_t s, int nsys) {
+  StageArgs a = a0;
+  a.tj = TJ4;
+  finalize_tiles(a);
   dim3 grid((unsigned)(a.nti * a.ntj * a.ntk), (unsigned)nsys, 1);
   dim3 block(TI, TJ4, 1);
   if (a.G.metric_ef)
@@ -1186,8 +1235,6 @@ hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
   StageArgs a = a0;
   const int nsys = a.nsys;
   if (a.G.fd4) {
-    a.tj = TJ4;
-    finalize_tiles(a);
     switch (mode) {
       case M_RK4_S1: return launch_fd4<M_RK4_S1>(a, s, nsys);
       case M_RK4_S2: return launch_fd4<M_RK4_S2>(a, s, nsys);

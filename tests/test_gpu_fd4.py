@@ -80,7 +80,7 @@ def test_fd4_launch_shape_does_not_change_results(kc):
     sim, gpu, u0 = _pair(p, "rk4", seed=5)
     sim.u[...] = u0
     gpu.set_u_soa(u0[None])
-    gpu.set_tuning(kc, 4)  # the fd4 stages run 8 theta rows per workgroup whatever tj says
+    gpu.set_tuning(kc, 4)  # the fd4 stages choose their own theta rows per workgroup
     sim.step("rk4", 4)
     gpu.step(4)
     assert np.array_equal(gpu.u_soa()[0], sim.u), kc
