@@ -85,7 +85,7 @@ def measured_traffic(tuning, operators):
     return prof["traffic_bytes_per_launch"]
 
 
-def batched_vs_plain(n, steps, tuning):
+def batched_vs_plain(n, steps, tuning, operators):
     """The uneven batched workload: n evolutions (Nr in [256, 2048], (Ntheta, Nphi)
     in {(16,32), (32,64), (64,128)}, m ~ U[0,0.5], M ~ U[0.5,2], RK4), advanced
     `steps` steps (a) all together, one launch per stage (kg_batch_step), and
@@ -97,7 +97,7 @@ def batched_vs_plain(n, steps, tuning):
     specs = uneven_specs(n)
     members = []
     for s in specs:
-        m = make_member(s, on_device=True)
+        m = make_member(s, on_device=True, operators=operators)
         m.set_tuning(*tuning)
         members.append(m)
     torch.cuda.empty_cache()
@@ -121,7 +121,7 @@ def batched_vs_plain(n, steps, tuning):
     for m in members:
         m.close()
     vb, vp = pts * steps / tb, pts * steps / tp
-    return {"members": n, "steps": steps, "grid_points": pts, "batched_updates_per_s": vb,
+    return {"members": n, "steps": steps, "operators": operators, "grid_points": pts, "batched_updates_per_s": vb,
             "plain_updates_per_s": vp, "speedup": vb / vp,
             "workload": "uneven RK4 batch: Nr~U{256..2048}, (Ntheta,Nphi) in {(16,32),(32,64),(64,128)}, "
                         "m~U[0,0.5], M~U[0.5,2], seed 20261203",
@@ -181,7 +181,7 @@ def initial_data(ncart, reps=3):
             "amp": amp, "state": "complex, 5 fields on the full grid"}
 
 
-def serving(nreq, tuning):
+def serving(nreq, tuning, operators):
     """Serving loop: nreq requests of the batched workload arriving as a Poisson
     stream (50/s), steps ~U{500..5000}, admitted against the free HBM and run by
     continuous batching; each returns its final fields and its spectrum.  The
@@ -190,7 +190,7 @@ def serving(nreq, tuning):
     from kg_amd.serving import GpuEngine, ServingLoop, percentile, poisson_requests
 
     reqs = poisson_requests(nreq)
-    eng = GpuEngine(tuning=tuning)
+    eng = GpuEngine(tuning=tuning, operators=operators)
     free, _ = torch.cuda.mem_get_info()
     budget = int(0.85 * free)
     results, stats = ServingLoop(eng, budget, chunk=50).run(reqs, keep_fields=True)
@@ -201,7 +201,7 @@ def serving(nreq, tuning):
         eng.run_alone(r, keep_fields=True)
     tp = time.perf_counter() - t0
     ups_plain = stats.updates / tp
-    return {"requests": nreq, "arrival_rate_per_s": 50.0, "hbm_budget_bytes": budget,
+    return {"requests": nreq, "operators": operators, "arrival_rate_per_s": 50.0, "hbm_budget_bytes": budget,
             "p50_time_to_result_s": percentile(lat, 50), "p99_time_to_result_s": percentile(lat, 99),
             "served_updates_per_s": stats.updates_per_s, "one_at_a_time_updates_per_s": ups_plain,
             "speedup": stats.updates_per_s / ups_plain, "max_active": stats.max_active,
@@ -338,14 +338,15 @@ def main():
     if args.ic > 0:
         print("bench: initial data, ncart %d" % args.ic, file=sys.stderr, flush=True)
         out["initial_data"] = initial_data(args.ic)
-    best_ref2 = tuple(out["plain_ref2"]["tuning"].values()) if "plain_ref2" in out else (
-        tuple(head["tuning"].values()) if args.operators == "ref2" else (8, 8))
+    # the batched and serving legs run the timed workload's operator set; their
+    # grids are far smaller than the plain one, so kc = 8 for every member
+    shape = (8, 8) if args.operators == "fd4" else tuple(head["tuning"].values())
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
-        out["batched"] = batched_vs_plain(args.batched, args.batched_steps, best_ref2)
+        out["batched"] = batched_vs_plain(args.batched, args.batched_steps, shape, args.operators)
     if args.serve > 0:
         print("bench: serving loop, %d requests" % args.serve, file=sys.stderr, flush=True)
-        out["serving"] = serving(args.serve, best_ref2)
+        out["serving"] = serving(args.serve, shape, args.operators)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.operators)

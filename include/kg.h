@@ -67,7 +67,7 @@ enum {
  * KG_OPS_FD4: the 4th-order set the same file defines (x_4 :124-151, y_4
  * :241-268, z_4 :340-367, advec_x_4 :922-957) in every first derivative of
  * lines_method_lib.f90:671-753 and in the metric's d_r beta^r; for RK3 and
- * RK4 (the PIRK schemes and kg_batch_step return KG_ERR_UNSUPPORTED). */
+ * RK4 (the PIRK schemes return KG_ERR_UNSUPPORTED). */
 enum { KG_OPS_REF2 = 0, KG_OPS_FD4 = 1 };
 
 /* Descriptor: the arguments of lines_method_constructor
@@ -146,7 +146,7 @@ kg_status kg_step(kg_handle* h, int32_t nsteps);
  * per process).  Advances every handle of hs[0..n) by nsteps RK4 steps, each
  * stage one kernel launch over the tiles of all grids (grids may differ in
  * size, mass, black-hole parameters and initial data; all must be RK4 with
- * the same metric form).  Work runs on hs[0]'s stream; later calls on any
+ * the same metric form and operator set).  Work runs on hs[0]'s stream; later calls on any
  * member are ordered after it.  Each member's result equals stepping it
  * alone. */
 kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);

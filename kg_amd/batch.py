@@ -72,11 +72,11 @@ def uneven_specs(n: int = 256, seed: int = 20261203):
     return out
 
 
-def make_member(spec: MemberSpec, CFL: float = 0.25, on_device: bool = False) -> LinesMethod:
+def make_member(spec: MemberSpec, CFL: float = 0.25, on_device: bool = False, operators: str = "ref2") -> LinesMethod:
     """Create the grid on the GPU and upload its ghost-filled initial data
     (built on the host, or with torch on the GPU when on_device)."""
     lm = LinesMethod(Nx=spec.Nr, Ny=spec.Ntheta + 1, Nz=spec.Nphi + 1, xmin=spec.xmin, xmax=spec.xmax, CFL=CFL,
-                     m=spec.m, M_bh=spec.M_bh, scheme="rk4")
+                     m=spec.m, M_bh=spec.M_bh, scheme="rk4", operators=operators)
     args = (spec.Nr, spec.Ntheta + 1, spec.Nphi + 1, 2, spec.xmin, spec.xmax, spec.r0, spec.width, 1.0, spec.coeffs)
     if on_device:
         import torch

@@ -455,7 +455,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
     kg_handle* h = hs[q];
     if (!h) return fail(lead, KG_ERR_INVALID, "batch: null handle %d", q);
     if (h->desc.scheme != KG_RK4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d is not RK4", q);
-    if (h->G.fd4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d uses the fd4 operator set", q);
+    if (h->G.fd4 != lead->G.fd4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed operator sets");
     if (h->G.metric_ef != ef) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed metric forms");
     if (h->device != lead->device) return fail(lead, KG_ERR_INVALID, "batch: handles on different devices");
     for (int p = 0; p < q; ++p)
@@ -482,7 +482,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
       for (int st = 0; st < nst; ++st) {
         StageArgs a = make_stage_args(h, modes[st], fb, rl[st]);
         a.kc = lead->tune_kc;
-        a.tj = lead->tune_tj;
+        a.tj = lead->G.fd4 ? 8 : lead->tune_tj;  // the fd4 stages run 8 theta rows per workgroup
         finalize_tiles(a);
         sd[(v * nst + st) * n + q] = a;
       }
@@ -545,7 +545,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
         KG_HIP(lead, hipEventRecord(e0, lead->stream));
       }
       KG_HIP(lead, launch_stage_batch(modes[st], dsd + (size_t)(v * nst + st) * n, dts, n, tstart[n],
-                                      lead->tune_tj, ef, lead->stream));
+                                      lead->tune_tj, ef, lead->G.fd4, lead->stream));
       if (lead->prof) {
         KG_HIP(lead, hipEventRecord(e1, lead->stream));
         lead->ev_marks.push_back({0, lead->ev_used - 2});

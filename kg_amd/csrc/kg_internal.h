@@ -126,7 +126,7 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase /*0 x,1 y,2 z
 // systems (finalize_tiles sets the tile counts).  RK4 stages only.
 void finalize_tiles(StageArgs& a);
 hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int tj,
-                              int ef, hipStream_t s);
+                              int ef, int fd4, hipStream_t s);
 // Fill pass `phase`: grid q owns blocks [bstart[q], ...) of 256 work items, nitems[q] items.
 int64_t fill_work_items(const Geom& G, int nsys, int phase);
 hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrids, const int64_t* nitems,

@@ -1023,6 +1023,25 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel(const StageArgs a) {
   stage4_body<MODE, EF>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
 }
 
+// Many fd4 grids in one launch (as stage_kernel_batch).
+template <int MODE, int EF>
+__global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* __restrict__ descs,
+                                                                const int* __restrict__ tstart, int ngrids, int total) {
+  const int b = remap_xcd((int)blockIdx.x, total);
+  int lo = 0, hi = ngrids - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tstart[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const StageArgs& a = descs[lo];
+  const int local = b - tstart[lo];
+  const int nt = a.nti * a.ntj * a.ntk;
+  stage4_body<MODE, EF>(a, local % nt, local / nt);
+}
+
 // One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The
 // second launch bound is the minimum number of waves per SIMD.  RK4 stages 2
 // and 3 ask for 4 (128 VGPRs; 2 over their natural allocation).  The final
@@ -1215,8 +1234,28 @@ void finalize_tiles(StageArgs& a) {
   a.ntk = (a.k_hi - a.k_lo + a.kc - 1) / a.kc;
 }
 
+template <int MODE>
+hipError_t launch_batch4(const StageArgs* descs, const int* tstart, int ngrids, int total, int ef, hipStream_t s) {
+  dim3 grid((unsigned)total, 1, 1);
+  dim3 block(TI, TJ4, 1);
+  if (ef)
+    hipLaunchKernelGGL((stage4_kernel_batch<MODE, 1>), grid, block, 0, s, descs, tstart, ngrids, total);
+  else
+    hipLaunchKernelGGL((stage4_kernel_batch<MODE, 0>), grid, block, 0, s, descs, tstart, ngrids, total);
+  return hipGetLastError();
+}
+
 hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int tj,
-                              int ef, hipStream_t s) {
+                              int ef, int fd4, hipStream_t s) {
+  if (fd4) {
+    switch (mode) {
+      case M_RK4_S1: return launch_batch4<M_RK4_S1>(descs, tstart, ngrids, total, ef, s);
+      case M_RK4_S2: return launch_batch4<M_RK4_S2>(descs, tstart, ngrids, total, ef, s);
+      case M_RK4_S3: return launch_batch4<M_RK4_S3>(descs, tstart, ngrids, total, ef, s);
+      case M_RK4_S4: return launch_batch4<M_RK4_S4>(descs, tstart, ngrids, total, ef, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   const bool t8 = tj == 8;
   switch (mode) {
     case M_RK4_S1: return t8 ? launch_batch_tj<M_RK4_S1, 8>(descs, tstart, ngrids, total, ef, s)

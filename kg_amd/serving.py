@@ -172,10 +172,11 @@ class GpuEngine:
     projector; ``step`` is one batched launch per RK4 stage."""
 
     def __init__(self, lmax: int = 8, nk: int = 512, kmax: float = 4.0, CFL: float = 0.25, tuning=None,
-                 on_device: bool = True):
+                 on_device: bool = True, operators: str = "ref2"):
         self.lmax, self.nk, self.kmax, self.CFL = lmax, nk, kmax, CFL
         self.tuning = tuning
         self.on_device = on_device
+        self.operators = operators
 
     def bytes_for(self, spec: MemberSpec) -> int:
         """Device footprint: four RK4 state buffers (u, next state / k1, k2, k3),
@@ -194,7 +195,7 @@ class GpuEngine:
         from .batch import make_member
         from .spectral import SpectralProjector
 
-        lm = make_member(req.spec, CFL=self.CFL, on_device=self.on_device)
+        lm = make_member(req.spec, CFL=self.CFL, on_device=self.on_device, operators=self.operators)
         if self.tuning:
             lm.set_tuning(*self.tuning)
         sp = SpectralProjector(lm, lmax=self.lmax, nk=self.nk, basis="uniform", kmax=self.kmax)

@@ -51,3 +51,31 @@ def test_batch_matches_oracle_and_interleaves_with_single_steps():
         sim.u[...] = u0[q]
         sim.step("rk4", nsteps[q])
         assert np.array_equal(batch[q].u_soa()[0], sim.u), q
+
+
+def test_fd4_batch_equals_each_member_alone_and_the_oracle():
+    """The 4th-order operator set in batched launches: every member equals its
+    solo run and the oracle, bit for bit; a batch may not mix operator sets."""
+    specs = _specs()
+    batch = [make_member(s, operators="fd4") for s in specs]
+    alone = [make_member(s, operators="fd4") for s in specs]
+    u0 = [b.u_soa()[0] for b in batch]
+    print("fd4 batch", flush=True)
+    batch_step(batch, 3)
+    batch[1].step(1)
+    batch_step(batch, 2)
+    for a in alone:
+        a.step(5)
+    alone[1].step(1)
+    for q, s in enumerate(specs):
+        assert np.array_equal(batch[q].u_soa(), alone[q].u_soa()), q
+        sim = oracle.Sim(s.Nr, s.Ntheta + 1, s.Nphi + 1, 2, s.xmin, s.xmax, "Kerr_Schild", 0.0, s.M_bh, s.m,
+                         "outflow", 0.25)
+        sim.set_order(4)
+        sim.u[...] = u0[q]
+        sim.step("rk4", 6 if q == 1 else 5)
+        assert np.array_equal(batch[q].u_soa()[0], sim.u), q
+    from kg_amd import KGError
+    mixed = [make_member(specs[0], operators="fd4"), make_member(specs[1])]
+    with pytest.raises(KGError):
+        batch_step(mixed, 1)

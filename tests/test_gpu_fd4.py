@@ -162,8 +162,4 @@ def test_fd4_rejections():
     with pytest.raises(KGError) as ei:
         lm.pirk2()
     assert ei.value.status == capi.KG_ERR_UNSUPPORTED
-    from kg_amd.batch import batch_step
-    with pytest.raises(KGError) as ei:
-        batch_step([lm], 1)
-    assert ei.value.status == capi.KG_ERR_UNSUPPORTED
     lm.close()
