@@ -2,7 +2,7 @@
 ! loop (main/evolve.f90:22-26), with the step done by the MI355X engine.
 !
 ! usage: kg_evolve Nx Ny Nz xmin xmax m scheme nsteps in.bin out.bin
-!   scheme: pirk2 | pirk3 | rk3 | rk4
+!   scheme: pirk2 | pirk3 | rk3 | rk4, or rk3_fd4 | rk4_fd4 for the 4th-order stencils
 !   in.bin/out.bin: u(5, -g:Nx+g, -g:Ny+g, 0:Nz) complex(8), raw stream
 !   in.bin may instead be ic=<type>,<mu>,<sigma>,<ncart>,<seed>: the initial
 !   data of main/initial_conditions.f90 built by the engine
@@ -12,7 +12,7 @@ program kg_evolve
   type(kg_lines_method) :: MoL
   complex(kind=8), allocatable :: u(:,:,:,:)
   character(len=256) :: arg, fin, fout, scheme
-  integer :: Nx, Ny, Nz, nsteps, l, g, isch, ncart
+  integer :: Nx, Ny, Nz, nsteps, l, g, isch, ncart, ops
   integer(kind=8) :: seed
   real(kind=8) :: xmin, xmax, m, mu, sigma, amp
   character(len=64) :: ictype
@@ -29,15 +29,18 @@ program kg_evolve
   call get_command_argument(9, fin)
   call get_command_argument(10, fout)
 
+  ops = KG_OPS_REF2
   select case (trim(scheme))
   case ('pirk2'); isch = KG_SCHEME_PIRK2
   case ('pirk3'); isch = KG_SCHEME_PIRK3
   case ('rk3');   isch = KG_SCHEME_RK3
+  case ('rk3_fd4'); isch = KG_SCHEME_RK3; ops = KG_OPS_FD4
+  case ('rk4_fd4'); isch = KG_SCHEME_RK4; ops = KG_OPS_FD4
   case default;   isch = KG_SCHEME_RK4
   end select
 
   MoL = kg_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
-                                    'outflow', 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true.)
+                                    'outflow', 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., ops)
   allocate(u(5, -g:Nx+g, -g:Ny+g, 0:Nz))
   if (fin(1:3) == 'ic=') then
     read(fin(4:), *) ictype, mu, sigma, ncart, seed

@@ -22,6 +22,7 @@ module kg_hip_mod
                                            KG_METRIC_MKS = 2, KG_METRIC_SCHW_EF = 3
   integer(c_int32_t), parameter, public :: KG_BC_OUTFLOW = 0, KG_BC_INFLOW = 1
   integer(c_int32_t), parameter, public :: KG_LAYOUT_FORTRAN = 0, KG_LAYOUT_SOA = 1
+  integer(c_int32_t), parameter, public :: KG_OPS_REF2 = 0, KG_OPS_FD4 = 1
 
   type, bind(C), public :: kg_desc
     integer(c_int32_t) :: nvars, g_pts, Nx, Ny, Nz
@@ -189,13 +190,16 @@ contains
 
   ! lines_method_constructor(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, tmax,
   ! CFL, boundary_type_rmax, metric, a, M_bh, m) -- lines_method_lib.f90:49-129
+  ! op_set (optional): KG_OPS_REF2 (default, the reference's stencils) or
+  ! KG_OPS_FD4 (its 4th-order set; rk3 and rk4)
   function kg_lines_method_constructor(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, tmax, CFL, &
-                                       boundary_type_rmax, metric, a, M_bh, m, scheme, is_complex) result(this)
+                                       boundary_type_rmax, metric, a, M_bh, m, scheme, is_complex, op_set) result(this)
     integer, intent(in) :: nvars, g_pts, Nx, Ny, Nz
     real(kind=8), intent(in) :: xmin, xmax, tmin, tmax, CFL, a, M_bh, m
     character(len=*), intent(in) :: boundary_type_rmax, metric
     integer, intent(in) :: scheme
     logical, intent(in) :: is_complex
+    integer, intent(in), optional :: op_set
     type(kg_lines_method) :: this
     type(kg_desc) :: d
     real(c_double) :: t, dt
@@ -220,7 +224,8 @@ contains
       stop
     end if
     d%is_complex = merge(1, 0, is_complex)
-    d%op_set = 0
+    d%op_set = KG_OPS_REF2
+    if (present(op_set)) d%op_set = int(op_set, c_int32_t)
     d%xmin = xmin; d%xmax = xmax; d%tmin = tmin; d%CFL = CFL; d%dt = 0.0d0
     d%a = a; d%M_bh = M_bh; d%m = m
     call kg_check(kg_create(d, this%h), c_null_ptr)

@@ -19,7 +19,7 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built (no amdflang)")]
 
 
-@pytest.mark.parametrize("scheme", ["pirk2", "rk4"])
+@pytest.mark.parametrize("scheme", ["pirk2", "rk4", "rk4_fd4"])
 def test_fortran_driver_matches_oracle(tmp_path, scheme):
     Nx, Ny, Nz, xmin, xmax, m = 20, 9, 17, 1.8, 10.0, 0.2
     re = shell_state(Nx, Ny, Nz, 2, xmin, xmax, 5.0, 1.2, 1.0, {(2, 0): 1.0, (1, -1): 0.4})
@@ -37,8 +37,10 @@ def test_fortran_driver_matches_oracle(tmp_path, scheme):
     got = np.transpose(got, (0, 3, 2, 1))
     for part, arr in (("re", re), ("im", im)):
         sim = oracle.Sim(Nx, Ny, Nz, 2, xmin, xmax, "Kerr_Schild", 0.0, 1.0, m, "outflow", 0.25)
+        if scheme.endswith("_fd4"):
+            sim.set_order(4)
         sim.u[...] = arr
-        sim.step(scheme, nsteps)
+        sim.step(scheme.replace("_fd4", ""), nsteps)
         g = got.real if part == "re" else got.imag
         assert np.array_equal(g, sim.u), (scheme, part)
 
