@@ -59,6 +59,8 @@ def _load():
     lib.kgo_advec_x_4_raw.argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp, _dp]
     lib.kgo_set_order.argtypes = [C.c_void_p, C.c_int]
     lib.kgo_set_order.restype = C.c_int
+    lib.kgo_threads.restype = C.c_int
+    lib.kgo_threads.argtypes = []
     for nm in ("kgo_rhs_phi", "kgo_rhs_psi1", "kgo_rhs_psi2", "kgo_rhs_psi3", "kgo_rhs_pi", "kgo_rhs_all"):
         getattr(lib, nm).argtypes = [C.c_void_p, _dp, _dp]
     lib.kgo_fill.argtypes = [C.c_void_p, _dp, _dp]
@@ -80,6 +82,11 @@ def lib():
 def _ptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_dp)
+
+
+def threads() -> int:
+    """Host threads the oracle's point loops use (OMP_NUM_THREADS)."""
+    return lib().kgo_threads()
 
 
 def reference_dt(Nx, Ny, Nz, g, xmin, xmax, CFL):

@@ -20,6 +20,24 @@
 #define IDX(s, ii, jj, kk) (((int64_t)(kk) * (s)->nj + (jj)) * (s)->ni + (ii))
 #define IDX2(s, ii, jj) ((int64_t)(jj) * (s)->ni + (ii))
 
+/* Loops over independent points run on every host thread (OpenMP); each
+ * point's arithmetic is the same on any thread, so results do not depend on
+ * the thread count. */
+#ifdef _OPENMP
+#include <omp.h>
+#define KGO_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define KGO_PARALLEL_FOR
+#endif
+
+int kgo_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 static double* alloc0(int64_t n) {
   double* p = (double*)calloc((size_t)n, sizeof(double));
   return p;
@@ -242,6 +260,7 @@ static void metric_build(kgo_sim* s) {
 void kgo_dx1_raw(int ni, int nj, int nk, double h, const double* f, double* df) {
   /* first_derivative_x_2_uniform, finite_differences.f90:36-42 */
   double res = 0.50 / h;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < nk; ++kk)
     for (int jj = 0; jj < nj; ++jj) {
       const double* a = f + ((int64_t)kk * nj + jj) * ni;
@@ -260,6 +279,7 @@ void kgo_dx2_raw(int ni, int nj, int nk, double h, const double* f, double* df) 
   int Nz = nk - 1;
   int dis = Nz / 2;
   const int64_t pl = (int64_t)ni * nj;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < nk; ++kk) {
     int kt = (kk + dis) % (Nz - 1);
     const double* a = f + kk * pl;
@@ -282,15 +302,19 @@ void kgo_dx3_raw(int ni, int nj, int nk, double h, const double* f, double* df) 
   double res = 0.50 / h;
   const int64_t pl = (int64_t)ni * nj;
   int lo = 0, hi = nk - 1;
+  KGO_PARALLEL_FOR
   for (int64_t p = 0; p < pl; ++p) df[lo * pl + p] = (-f[(hi - 2) * pl + p] + f[(lo + 1) * pl + p]) * res;
+  KGO_PARALLEL_FOR
   for (int k = lo + 1; k <= hi - 1; ++k)
     for (int64_t p = 0; p < pl; ++p) df[k * pl + p] = (-f[(k - 1) * pl + p] + f[(k + 1) * pl + p]) * res;
+  KGO_PARALLEL_FOR
   for (int64_t p = 0; p < pl; ++p) df[hi * pl + p] = (-f[(lo + 2) * pl + p] + f[(hi - 1) * pl + p]) * res;
 }
 
 void kgo_advec_x_raw(int ni, int nj, int nk, double h, const double* f, const double* beta, double* df) {
   /* advec_x_uniform, finite_differences.f90:779-799; beta is a full 3-D array */
   double res = 0.5 / h;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < nk; ++kk)
     for (int jj = 0; jj < nj; ++jj) {
       const int64_t o = ((int64_t)kk * nj + jj) * ni;
@@ -332,6 +356,7 @@ void kgo_dx1_4_raw(int ni, int nj, int nk, double h, const double* f, double* df
   /* first_derivative_x_4 (:124-151): res = 1/(12h), N = size(f,1)-1 */
   const double res = 1.0 / (12.0 * h);
   const int N = ni - 1;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < nk; ++kk)
     for (int jj = 0; jj < nj; ++jj) {
       const double* a = f + ((int64_t)kk * nj + jj) * ni;
@@ -348,6 +373,7 @@ void kgo_dx2_4_raw(int ni, int nj, int nk, double h, const double* f, double* df
   /* first_derivative_y_4 (:241-268): N = size(f,2)-1; no pole rows */
   const double res = 1.0 / (12.0 * h);
   const int N = nj - 1;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < nk; ++kk)
     for (int ii = 0; ii < ni; ++ii) {
       const double* a = f + (int64_t)kk * nj * ni + ii;
@@ -365,6 +391,7 @@ void kgo_dx3_4_raw(int ni, int nj, int nk, double h, const double* f, double* df
   const double res = 1.0 / (12.0 * h);
   const int N = nk - 1;
   const int64_t pl = (int64_t)ni * nj;
+  KGO_PARALLEL_FOR
   for (int64_t p = 0; p < pl; ++p) {
     const double* a = f + p;
     double* d = df + p;
@@ -376,31 +403,33 @@ void kgo_dx3_4_raw(int ni, int nj, int nk, double h, const double* f, double* df
   }
 }
 
+/* advec_x_4 on one r-row a[0..N] with the row's beta b[0..N] */
+static void advec4_row(const double* a, const double* b, double* d, int N, double res) {
+  d[0] = (-25.0 * a[0] + 48.0 * a[1] - 36.0 * a[2] + 16.0 * a[3] - 3.0 * a[4]) * res / 12.0;
+  d[1] = (-3.0 * a[0] - 10.0 * a[1] + 18.0 * a[2] - 6.0 * a[3] + a[4]) * res / 12.0;
+  d[2] = (a[0] - 8.0 * a[1] + 8.0 * a[3] - a[4]) * res / 12.0;
+  for (int i = 3; i <= N - 3; ++i) {
+    if (b[i] < 0.0)
+      d[i] = (3.0 * a[i + 1] + 10.0 * a[i] - 18.0 * a[i - 1] + 6.0 * a[i - 2] - a[i - 3]) * res / 12.0;
+    else if (b[i] > 0.0)
+      d[i] = (-3.0 * a[i - 1] - 10.0 * a[i] + 18.0 * a[i + 1] - 6.0 * a[i + 2] + a[i + 3]) * res / 12.0;
+    else
+      d[i] = (a[i - 2] - 8.0 * a[i - 1] + 8.0 * a[i + 1] - a[i + 2]) * res / 12.0;
+  }
+  d[N - 2] = (a[N - 4] - 8.0 * a[N - 3] + 8.0 * a[N - 1] - a[N]) * res / 12.0;
+  d[N - 1] = (3.0 * a[N] + 10.0 * a[N - 1] - 18.0 * a[N - 2] + 6.0 * a[N - 3] - a[N - 4]) * res / 12.0;
+  d[N] = (25.0 * a[N] - 48.0 * a[N - 1] + 36.0 * a[N - 2] - 16.0 * a[N - 3] + 3.0 * a[N - 4]) * res / 12.0;
+}
+
 void kgo_advec_x_4_raw(int ni, int nj, int nk, double h, const double* f, const double* beta, double* df) {
   /* advec_x_4 (:922-957): res = 1/dx, every row (...)*res/12; rows 0, 1, 2,
    * N-2, N-1, N fixed; 3..N-3 upwinded by the sign of beta (beta < 0 first) */
   const double res = 1.0 / h;
-  const int N = ni - 1;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < nk; ++kk)
     for (int jj = 0; jj < nj; ++jj) {
       const int64_t o = ((int64_t)kk * nj + jj) * ni;
-      const double* a = f + o;
-      const double* b = beta + o;
-      double* d = df + o;
-      d[0] = (-25.0 * a[0] + 48.0 * a[1] - 36.0 * a[2] + 16.0 * a[3] - 3.0 * a[4]) * res / 12.0;
-      d[1] = (-3.0 * a[0] - 10.0 * a[1] + 18.0 * a[2] - 6.0 * a[3] + a[4]) * res / 12.0;
-      d[2] = (a[0] - 8.0 * a[1] + 8.0 * a[3] - a[4]) * res / 12.0;
-      for (int i = 3; i <= N - 3; ++i) {
-        if (b[i] < 0.0)
-          d[i] = (3.0 * a[i + 1] + 10.0 * a[i] - 18.0 * a[i - 1] + 6.0 * a[i - 2] - a[i - 3]) * res / 12.0;
-        else if (b[i] > 0.0)
-          d[i] = (-3.0 * a[i - 1] - 10.0 * a[i] + 18.0 * a[i + 1] - 6.0 * a[i + 2] + a[i + 3]) * res / 12.0;
-        else
-          d[i] = (a[i - 2] - 8.0 * a[i - 1] + 8.0 * a[i + 1] - a[i + 2]) * res / 12.0;
-      }
-      d[N - 2] = (a[N - 4] - 8.0 * a[N - 3] + 8.0 * a[N - 1] - a[N]) * res / 12.0;
-      d[N - 1] = (3.0 * a[N] + 10.0 * a[N - 1] - 18.0 * a[N - 2] + 6.0 * a[N - 3] - a[N - 4]) * res / 12.0;
-      d[N] = (25.0 * a[N] - 48.0 * a[N - 1] + 36.0 * a[N - 2] - 16.0 * a[N - 3] + 3.0 * a[N - 4]) * res / 12.0;
+      advec4_row(f + o, beta + o, df + o, ni - 1, res);
     }
 }
 
@@ -457,11 +486,13 @@ static void op_adv(const kgo_sim* s, const double* f, double* df) {
     return;
   }
   /* advec_x_4 with beta(i,j,k) = beta2d(i,j) */
-  double* b3 = (double*)malloc(sizeof(double) * (size_t)s->npts);
-  const int64_t n2 = (int64_t)s->ni * s->nj;
-  for (int kk = 0; kk < s->nk; ++kk) memcpy(b3 + kk * n2, s->beta_r, sizeof(double) * (size_t)n2);
-  kgo_advec_x_4_raw(s->ni, s->nj, s->nk, s->dx, f, b3, df);
-  free(b3);
+  const double res = 1.0 / s->dx;
+  KGO_PARALLEL_FOR
+  for (int kk = 0; kk < s->nk; ++kk)
+    for (int jj = 0; jj < s->nj; ++jj) {
+      const int64_t o = ((int64_t)kk * s->nj + jj) * s->ni;
+      advec4_row(f + o, s->beta_r + IDX2(s, 0, jj), df + o, s->ni - 1, res);
+    }
 }
 
 void kgo_dx1(const kgo_sim* s, const double* f, double* df) { kgo_dx1_raw(s->ni, s->nj, s->nk, s->dx, f, df); }
@@ -472,6 +503,7 @@ void kgo_advec_x(const kgo_sim* s, const double* f, const double* beta2d, double
   /* same as advec_x_uniform with beta(i,j,k) = beta2d(i,j) */
   double res = 0.5 / s->dx;
   const int ni = s->ni, nj = s->nj, nk = s->nk;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < nk; ++kk)
     for (int jj = 0; jj < nj; ++jj) {
       const int64_t o = ((int64_t)kk * nj + jj) * ni;
@@ -503,6 +535,7 @@ void kgo_advec_x(const kgo_sim* s, const double* f, const double* beta2d, double
 void kgo_rhs_phi(kgo_sim* s, double* rhs, const double* vars) {
   /* :675  rhs = alpha*vars(5) + beta(1)*vars(2) */
   const double *v2 = FIELD(vars, 1), *v5 = FIELD(vars, 4);
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
@@ -515,6 +548,7 @@ void kgo_rhs_psi1(kgo_sim* s, double* rhs, const double* vars) {
   /* :687-689  rhs = dx1(alpha*vars(5)) + beta*ad_1(vars(2),beta) + dx_beta*vars(2) */
   const double *v2 = FIELD(vars, 1), *v5 = FIELD(vars, 4);
   double *t0 = s->tmp[0], *t1 = s->tmp[1], *t2 = s->tmp[2];
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
@@ -523,6 +557,7 @@ void kgo_rhs_psi1(kgo_sim* s, double* rhs, const double* vars) {
       }
   op_dr(s, t0, t1);
   op_adv(s, v2, t2);
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
@@ -534,6 +569,7 @@ void kgo_rhs_psi1(kgo_sim* s, double* rhs, const double* vars) {
 static void flux_F(kgo_sim* s, const double* vars, double* F) {
   /* alpha*vars(5) + beta(1)*vars(2)  (:700, :711) */
   const double *v2 = FIELD(vars, 1), *v5 = FIELD(vars, 4);
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
@@ -563,6 +599,7 @@ void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars) {
   double *fr = s->tmp[0], *ft = s->tmp[1], *fp = s->tmp[2], *ps = s->tmp[3];
   double *d1 = s->tmp[4], *d2 = s->tmp[5], *d3 = s->tmp[6], *ad = s->tmp[7];
   const int ef = (s->metric == KGO_METRIC_SCHW_EF);
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
@@ -583,6 +620,7 @@ void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars) {
   op_dph(s, fp, d3);
   op_adv(s, ps, ad);
   const double m2 = s->m * s->m;
+  KGO_PARALLEL_FOR
   for (int kk = 0; kk < s->nk; ++kk)
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
@@ -685,8 +723,10 @@ static void pirk_stage1(const kgo_ops* ops, kgo_bufs* b, double dt, double* st) 
   const int64_t n = ops->n;
   ops->rhs14(ops->ctx, b->u, b->rhs_u);
   for (int c = 0; c < 4; ++c)
+    KGO_PARALLEL_FOR
     for (int64_t p = 0; p < n; ++p) st[c * n + p] = b->u[c * n + p] + dt * b->rhs_u[c * n + p];
   ops->rhs5(ops->ctx, st, b->rhs_u + 4 * n);
+  KGO_PARALLEL_FOR
   for (int64_t p = 0; p < n; ++p) st[4 * n + p] = b->u[4 * n + p] + dt * b->rhs_u[4 * n + p];
 }
 
@@ -697,6 +737,7 @@ void kgo_pirk2_generic(const kgo_ops* ops, kgo_bufs* b, double dt) {
   if (ops->fill) ops->fill(ops->ctx, b->u, b->u_p);                    /* :291 */
   ops->rhs14(ops->ctx, b->u_stage, b->rhs_u);                          /* :294-297 */
   ops->rhs5(ops->ctx, b->u_stage, b->rhs_u + 4 * n);                   /* :298 */
+  KGO_PARALLEL_FOR
   for (int64_t p = 0; p < n5; ++p)                                     /* :300-304 */
     b->u[p] = 0.5 * (b->u_p[p] + b->u_stage[p] + dt * b->rhs_u[p]);
   if (ops->fill) ops->fill(ops->ctx, b->u, b->u_p);                    /* :306 */
@@ -709,16 +750,19 @@ void kgo_pirk3_generic(const kgo_ops* ops, kgo_bufs* b, double dt) {
   if (ops->fill) ops->fill(ops->ctx, b->u, b->u_p);                    /* :353 */
   ops->rhs14(ops->ctx, b->u_stage, b->rhs_u);                          /* :356-359 */
   for (int c = 0; c < 4; ++c)                                          /* :361-364 */
+    KGO_PARALLEL_FOR
     for (int64_t p = 0; p < n; ++p)
       b->u_stage_2[c * n + p] =
           0.75 * b->u_p[c * n + p] + 0.25 * (b->u_stage[c * n + p] + dt * b->rhs_u[c * n + p]);
   ops->rhs5(ops->ctx, b->u_stage_2, b->rhs_u + 4 * n);                 /* :366 */
+  KGO_PARALLEL_FOR
   for (int64_t p = 0; p < n; ++p)                                      /* :367 */
     b->u_stage_2[4 * n + p] = 0.75 * b->u_p[4 * n + p] + 0.25 * (b->u_stage[4 * n + p] + dt * b->rhs_u[4 * n + p]);
   if (ops->fill) ops->fill(ops->ctx, b->u, b->u_p);                    /* :369 */
   ops->rhs14(ops->ctx, b->u_stage_2, b->rhs_u);                        /* :372-375 */
   ops->rhs5(ops->ctx, b->u_stage_2, b->rhs_u + 4 * n);                 /* :376 */
   const double third = 1.0 / 3.0, two_thirds = 2.0 / 3.0;
+  KGO_PARALLEL_FOR
   for (int64_t p = 0; p < n5; ++p)                                     /* :378-382 */
     b->u[p] = third * b->u_p[p] + two_thirds * (b->u_stage_2[p] + dt * b->rhs_u[p]);
   if (ops->fill) ops->fill(ops->ctx, b->u, b->u_p);                    /* :384 */
@@ -731,10 +775,13 @@ void kgo_rk3_generic(const kgo_ops* ops, kgo_bufs* b, double dt) {
     ops->rhs14(ops->ctx, b->u, b->rhs_u);                              /* rhs(1) */
     ops->rhs5(ops->ctx, b->u, b->rhs_u + 4 * n);
     if (rk == 1) {                                                     /* :229-237 */
+      KGO_PARALLEL_FOR
       for (int64_t p = 0; p < n5; ++p) b->u[p] = b->u_p[p] + b->rhs_u[p] * dt;
     } else if (rk == 2) {
+      KGO_PARALLEL_FOR
       for (int64_t p = 0; p < n5; ++p) b->u[p] = 0.750 * b->u_p[p] + 0.250 * (b->u[p] + b->rhs_u[p] * dt);
     } else {
+      KGO_PARALLEL_FOR
       for (int64_t p = 0; p < n5; ++p)
         b->u[p] = b->u_p[p] / 3.0 + 2.0 * (b->u[p] + b->rhs_u[p] * dt) / 3.0;
     }
@@ -749,21 +796,25 @@ void kgo_rk4_generic(const kgo_ops* ops, kgo_bufs* b, double dt) {
     ops->rhs14(ops->ctx, b->u, b->rhs_u);                              /* rhs(1) */
     ops->rhs5(ops->ctx, b->u, b->rhs_u + 4 * n);
     if (rk == 1) {                                                     /* :429-441 */
+      KGO_PARALLEL_FOR
       for (int64_t p = 0; p < n5; ++p) {
         b->k1[p] = b->rhs_u[p] * dt;
         b->u[p] = b->u_p[p] + 0.50 * b->k1[p];
       }
     } else if (rk == 2) {
+      KGO_PARALLEL_FOR
       for (int64_t p = 0; p < n5; ++p) {
         b->k2[p] = b->rhs_u[p] * dt;
         b->u[p] = b->u_p[p] + 0.50 * b->k2[p];
       }
     } else if (rk == 3) {
+      KGO_PARALLEL_FOR
       for (int64_t p = 0; p < n5; ++p) {
         b->k3[p] = b->rhs_u[p] * dt;
         b->u[p] = b->u_p[p] + b->k3[p];
       }
     } else {
+      KGO_PARALLEL_FOR
       for (int64_t p = 0; p < n5; ++p) {
         b->k4[p] = b->rhs_u[p] * dt;
         b->u[p] = b->u_p[p] + (b->k1[p] + 2.0 * b->k2[p] + 2.0 * b->k3[p] + b->k4[p]) / 6.0;

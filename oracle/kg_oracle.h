@@ -82,6 +82,9 @@ double* kgo_field(kgo_sim* s, int which);   /* 0=u 1=u_p 2=u_stage 3=u_stage_2 4
 double* kgo_table(kgo_sim* s, int which);   /* 0 alpha 1 beta_r 2 dx_beta 3 sqrtg 4 g_rr 5 g_rp 6 g_tt 7 g_pp 8 g_dn 9 g_up 10 x 11 y 12 z */
 void kgo_info(const kgo_sim* s, int64_t* out8, double* outd8);
 
+/* host threads the independent-point loops run on (OpenMP; 1 without it) */
+int kgo_threads(void);
+
 /* reference-formula dt (diagnostic_lib.f90:101-104) */
 double kgo_reference_dt(int Nx, int Ny, int Nz, int g, double xmin, double xmax, double CFL);
 

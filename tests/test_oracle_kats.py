@@ -412,3 +412,30 @@ def test_fd4_rk4_self_convergence_is_fourth_order():
     for f in (0, 1, 4):
         q = np.abs(a[f] - b[f]).max() / np.abs(b[f] - c[f]).max()
         assert 13.0 < q < 19.0, (f, q)
+
+
+def test_oracle_results_do_not_depend_on_the_thread_count():
+    """The oracle's point loops run on OpenMP threads; every scheme and both
+    operator sets give the same bits on 1 thread as on all of them."""
+    import os
+    import subprocess
+    import sys
+    code = ("import hashlib, oracle\n"
+            "from kg_amd.initial_data import shell_state, random_lm_coeffs\n"
+            "h = hashlib.sha256()\n"
+            "for order, scheme in ((2, 'pirk2'), (2, 'rk4'), (4, 'rk4'), (4, 'rk3'), (2, 'pirk3')):\n"
+            "    s = oracle.Sim(30, 9, 13, 2, 1.8, 12.0, 'Kerr_Schild', 0.5, 1.0, 0.2, 'inflow', 0.25)\n"
+            "    s.set_order(order)\n"
+            "    s.u[...] = shell_state(30, 9, 13, 2, 1.8, 12.0, 6.0, 1.5, 1.0, random_lm_coeffs(3, 3, 0.3))\n"
+            "    s.step(scheme, 5)\n"
+            "    h.update(s.u.tobytes())\n"
+            "print(h.hexdigest())\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = []
+    for nt in ("1", "4"):
+        env = dict(os.environ, OMP_NUM_THREADS=nt, PYTHONPATH=root)
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, cwd=root,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        out.append(r.stdout.strip())
+    assert out[0] == out[1]
