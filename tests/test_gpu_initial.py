@@ -22,12 +22,12 @@ pytestmark = pytest.mark.gpu
 GRID = dict(Nx=24, Ny=9, Nz=9, g=2, xmin=1.8, xmax=8.0, m=0.5)
 
 
-def _pair(is_complex, bc="outflow", scheme="rk4", **kw):
+def _pair(is_complex, bc="outflow", scheme="rk4", metric="Kerr_Schild", a=0.0, **kw):
     p = dict(GRID)
     p.update(kw)
     lm = LinesMethod(g_pts=p["g"], xmin=p["xmin"], xmax=p["xmax"], Nx=p["Nx"], Ny=p["Ny"], Nz=p["Nz"], m=p["m"],
-                     boundary_type_rmax=bc, scheme=scheme, is_complex=is_complex, CFL=0.25)
-    s = oracle.Sim(p["Nx"], p["Ny"], p["Nz"], p["g"], p["xmin"], p["xmax"], "Kerr_Schild", 0.0, 1.0, p["m"], bc)
+                     boundary_type_rmax=bc, scheme=scheme, is_complex=is_complex, CFL=0.25, metric=metric, a=a)
+    s = oracle.Sim(p["Nx"], p["Ny"], p["Nz"], p["g"], p["xmin"], p["xmax"], metric, a, 1.0, p["m"], bc)
     return lm, s
 
 
@@ -87,6 +87,23 @@ def test_initial_data_matches_oracle(kind, is_complex, bc):
     _assert_fields_close(got, want)
     if not is_complex:
         assert np.all(want.imag == 0.0)
+    lm.close()
+
+
+@pytest.mark.parametrize("kind,metric,a", [("spherical_shell", "Schw_EF", 0.0), ("random_shell", "Kerr_Schild", 0.7)])
+def test_initial_data_other_backgrounds(kind, metric, a):
+    """The lapse, shift and sqrt(gamma) of the handle's metric enter Pi and the
+    normalising integrals (Schwarzschild in EF form, spinning Kerr-Schild)."""
+    print("initial data", kind, metric, a, flush=True)
+    xmin = 2.5 if metric == "Schw_EF" else 1.8
+    lm, s = _pair(True, metric=metric, a=a, xmin=xmin)
+    n, seed, mu, sigma = 32, 5, 5.0, 1.0
+    r_max = s.table("x")[-1]
+    amp = initial_conditions(lm, kind, mu, sigma, ncart=n, seed=seed)
+    field = oracle.random_field(n, r_max, 1.0, oracle.uniforms(seed, n)) if kind != "spherical_shell" else None
+    want, amp_o = oracle.initial_conditions(s, kind, mu, sigma, field, r_max)
+    assert amp == pytest.approx(amp_o, rel=1e-13)
+    _assert_fields_close(_state(lm), want)
     lm.close()
 
 
