@@ -88,7 +88,26 @@ def measured_traffic(tuning, operators):
     return prof["traffic_bytes_per_launch"]
 
 
-def batched_vs_plain(n, steps, tuning, operators):
+def small_specs(n, seed=20261204):
+    """The smallest class of the uneven population alone: Nr = 256, (Ntheta,
+    Nphi) = (16, 32), masses and pulses drawn as for the full population."""
+    import numpy as np
+    from kg_amd.batch import DR, MemberSpec
+    from kg_amd.initial_data import random_lm_coeffs
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m, M = float(rng.uniform(0.0, 0.5)), float(rng.uniform(0.5, 2.0))
+        xmin = 0.9 * 2.0 * M
+        xmax = xmin + 256 * DR
+        r0 = float(rng.uniform(xmin + 10.0, xmin + 0.5 * (xmax - xmin)))
+        sd = int(rng.integers(0, 2**31))
+        out.append(MemberSpec(256, 16, 32, m, M, xmin, xmax, r0, 1.5, random_lm_coeffs(sd), sd))
+    return out
+
+
+def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     """The uneven batched workload: n evolutions (Nr in [256, 2048], (Ntheta, Nphi)
     in {(16,32), (32,64), (64,128)}, m ~ U[0,0.5], M ~ U[0.5,2], RK4), advanced
     `steps` steps (a) all together, one launch per stage (kg_batch_step), and
@@ -97,7 +116,7 @@ def batched_vs_plain(n, steps, tuning, operators):
     import torch
     from kg_amd.batch import batch_step, make_member, uneven_specs
 
-    specs = uneven_specs(n)
+    specs = specs if specs is not None else uneven_specs(n)
     members = []
     for s in specs:
         m = make_member(s, on_device=True, operators=operators)
@@ -120,7 +139,7 @@ def batched_vs_plain(n, steps, tuning, operators):
         m.step(steps)
         m.sync()
         tp += time.perf_counter() - t0
-    spec_out = spectral_projection(members, tb / steps)
+    spec_out = spectral_projection(members, tb / steps) if spectral else None
     for m in members:
         m.close()
     vb, vp = pts * steps / tb, pts * steps / tp
@@ -260,8 +279,17 @@ def plain(operators, steps, warmup, kc=0, tj=8):
     stage_ms, stage_n = lm.profile_read(0)
     fill_ms, fill_n = lm.profile_read(1)
     lm.profile(False)
-    u = lm.u_soa()[0]
-    finite = bool(np.isfinite(u).all())
+    # the drop-in boundary hands the state over in the reference's layout
+    # u(5, i, j, k): one download and one upload, timed (PCIe, host buffers)
+    t2 = time.perf_counter()
+    uf = lm.u_fortran()
+    t3 = time.perf_counter()
+    lm.set_u_fortran(uf)
+    lm.sync()
+    t4 = time.perf_counter()
+    finite = bool(np.isfinite(uf).all())
+    host_bytes = uf.nbytes
+    del uf
     lm.close()
     npts = CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1)
     elapsed = t1 - t0
@@ -275,7 +303,11 @@ def plain(operators, steps, warmup, kc=0, tj=8):
                          "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(best, operators),
                          "kernel": KERNELS[operators], "avg_kernel_ms": avg_stage_s * 1e3, "launches": stage_n,
                          "bytes_per_launch": per_launch_bytes},
-            "fill_ms_per_step": fill_ms / steps}
+            "fill_ms_per_step": fill_ms / steps,
+            "host_transfer": {"bytes": host_bytes, "download_s": t3 - t2, "upload_s": t4 - t3,
+                              "pcie_inclusive_updates_per_s": npts * steps / (elapsed + (t3 - t2) + (t4 - t3)),
+                              "note": "state handed over once each way in the reference's u(5,i,j,k) layout "
+                                      "around the timed steps; not part of value"}}
 
 
 def main():
@@ -333,6 +365,7 @@ def main():
                    "tuning": head["tuning"]},
         "roofline": head["roofline"],
         "fill_ms_per_step": head["fill_ms_per_step"],
+        "host_transfer": head["host_transfer"],
     }
     other = "ref2" if args.operators == "fd4" else "fd4"
     if args.other_steps > 0:
@@ -347,6 +380,13 @@ def main():
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
         out["batched"] = batched_vs_plain(args.batched, args.batched_steps, shape, args.operators)
+        print("bench: batched vs plain, %d small members" % args.batched, file=sys.stderr, flush=True)
+        small = batched_vs_plain(args.batched, 50, shape, args.operators, specs=small_specs(args.batched),
+                                 spectral=False)
+        small.pop("spectral", None)
+        small["workload"] = ("the population's smallest class alone: %d members of Nr=256, (Ntheta,Nphi)=(16,32), "
+                             "m~U[0,0.5], M~U[0.5,2], 50 RK4 steps" % args.batched)
+        out["batched_small_members"] = small
     if args.serve > 0:
         print("bench: serving loop, %d requests" % args.serve, file=sys.stderr, flush=True)
         out["serving"] = serving(args.serve, shape, args.operators)

@@ -373,53 +373,29 @@ kg_status do_steps(kg_handle* h, int scheme, int nsteps) {
   return KG_OK;
 }
 
-// host layout conversion: Fortran u(5,ni,nj,nk) [complex pairs] <-> dense [sys][5][nk][nj][ni]
-void fortran_to_dense(const kg_handle* h, const double* src, double* dst) {
-  const Geom& G = h->G;
-  const int64_t npts = (int64_t)G.ni * G.nj * G.nk;
-  const int ns = h->nsys;
-  for (int64_t p = 0; p < npts; ++p)
-    for (int c = 0; c < 5; ++c)
-      for (int s = 0; s < ns; ++s) dst[((int64_t)s * 5 + c) * npts + p] = src[(p * 5 + c) * ns + s];
-}
-
-void dense_to_fortran(const kg_handle* h, const double* src, double* dst) {
-  const Geom& G = h->G;
-  const int64_t npts = (int64_t)G.ni * G.nj * G.nk;
-  const int ns = h->nsys;
-  for (int64_t p = 0; p < npts; ++p)
-    for (int c = 0; c < 5; ++c)
-      for (int s = 0; s < ns; ++s) dst[(p * 5 + c) * ns + s] = src[((int64_t)s * 5 + c) * npts + p];
-}
-
 kg_status xfer(kg_handle* h, const State5& st, double* host, int32_t layout, int to_dev) {
   if (!host) return fail(h, KG_ERR_INVALID, "null host pointer");
   if (layout != KG_LAYOUT_FORTRAN && layout != KG_LAYOUT_SOA) return fail(h, KG_ERR_INVALID, "bad layout %d", layout);
   const Geom& G = h->G;
   const size_t n = (size_t)G.ni * G.nj * G.nk * 5 * h->nsys;
-  std::vector<double> tmp;
-  double* dense_host = host;
-  if (layout == KG_LAYOUT_FORTRAN) {
-    tmp.resize(n);
-    dense_host = tmp.data();
-    if (to_dev) fortran_to_dense(h, host, dense_host);
-  }
+  // the host array crosses PCIe as it is; the device reorders it (for the
+  // Fortran layout, u(5,i,j,k) with the parts fastest, in the pack kernel)
+  const int fortran = layout == KG_LAYOUT_FORTRAN;
   void* dbuf = nullptr;
   KG_HIP(h, hipMalloc(&dbuf, n * sizeof(double)));
   kg_status rc = KG_OK;
   hipError_t e = hipSuccess;
   if (to_dev) {
-    e = hipMemcpyAsync(dbuf, dense_host, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
-    if (e == hipSuccess) e = launch_pack(G, (double*)dbuf, st.f, h->nsys, 1, h->stream);
+    e = hipMemcpyAsync(dbuf, host, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_pack(G, (double*)dbuf, st.f, h->nsys, 1, h->stream, fortran);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   } else {
-    e = launch_pack(G, (double*)dbuf, st.f, h->nsys, 0, h->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dense_host, dbuf, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    e = launch_pack(G, (double*)dbuf, st.f, h->nsys, 0, h->stream, fortran);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dbuf, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   }
   if (e != hipSuccess) rc = fail(h, KG_ERR_HIP, "state transfer failed: %s", hipGetErrorString(e));
   (void)hipFree(dbuf);
-  if (rc == KG_OK && !to_dev && layout == KG_LAYOUT_FORTRAN) dense_to_fortran(h, dense_host, host);
   return rc;
 }
 

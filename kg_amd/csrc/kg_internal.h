@@ -132,7 +132,9 @@ int64_t fill_work_items(const Geom& G, int nsys, int phase);
 hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrids, const int64_t* nitems,
                              int nblocks, int phase, hipStream_t s);
 // dense [sys][5][nk][nj][ni] (device) <-> pitched fields; to_dev = 1: dense -> fields
-hipError_t launch_pack(const Geom& G, double* dense, double* const fields[5], int nsys, int to_dev, hipStream_t s);
+// fortran = 1: the host array is the reference's u(5, i, j, k) [parts fastest]
+hipError_t launch_pack(const Geom& G, double* dense, double* const fields[5], int nsys, int to_dev, hipStream_t s,
+                       int fortran = 0);
 
 // What the spectral projector needs from a handle: its device, stream, grid
 // and the current state's field buffers (these move between steps for RK4).

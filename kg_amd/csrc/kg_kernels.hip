@@ -1165,8 +1165,11 @@ __global__ __launch_bounds__(256) void fill_kernel_batch(const FillArgs* __restr
 }
 
 // dense [sys][5][nk][nj][ni] <-> pitched fields
+// fortran = 1: src is the reference's u(5, i, j, k) (complex pairs when nsys =
+// 2: variable, then part, fastest) instead of the dense SoA
 __global__ __launch_bounds__(256) void pack_kernel(const Geom G, double* __restrict__ src, double* d0,
-                                                   double* d1, double* d2, double* d3, double* d4, int to_dev) {
+                                                   double* d1, double* d2, double* d3, double* d4, int to_dev,
+                                                   int fortran, int nsys) {
   const int nbx = (G.ni + 63) / 64;
   const int ii = (blockIdx.x % nbx) * 64 + threadIdx.x;
   const int jj = blockIdx.x / nbx;
@@ -1175,7 +1178,9 @@ __global__ __launch_bounds__(256) void pack_kernel(const Geom G, double* __restr
   const int c = blockIdx.z % 5;
   const int s = blockIdx.z / 5;
   double* dst[5] = {d0, d1, d2, d3, d4};
-  const int64_t dense = ((((int64_t)s * 5 + c) * G.nk + kk) * G.nj + jj) * G.ni + ii;
+  const int64_t p = ((int64_t)kk * G.nj + jj) * G.ni + ii;
+  const int64_t dense =
+      fortran ? (p * 5 + c) * nsys + s : ((((int64_t)s * 5 + c) * G.nk + kk) * G.nj + jj) * G.ni + ii;
   const int64_t pitched = (int64_t)s * G.sys_stride + (int64_t)kk * G.plane + (int64_t)jj * G.ld + ii;
   if (to_dev)
     dst[c][pitched] = src[dense];
@@ -1330,9 +1335,10 @@ hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrid
 
 int64_t fill_work_items(const Geom& G, int nsys, int phase) { return fill_items(G, nsys, phase); }
 
-hipError_t launch_pack(const Geom& G, double* src, double* const d[5], int nsys, int to_dev, hipStream_t s) {
+hipError_t launch_pack(const Geom& G, double* src, double* const d[5], int nsys, int to_dev, hipStream_t s,
+                       int fortran) {
   hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((G.ni + 63) / 64 * G.nj), (unsigned)G.nk, 5 * nsys), dim3(64),
-                     0, s, G, src, d[0], d[1], d[2], d[3], d[4], to_dev);
+                     0, s, G, src, d[0], d[1], d[2], d[3], d[4], to_dev, fortran, nsys);
   return hipGetLastError();
 }
 
