@@ -67,6 +67,8 @@ struct kg_handle {
   int64_t prof_n[2] = {0, 0};
   void* batch_dev = nullptr;  // descriptors of batched launches led by this handle
   size_t batch_cap = 0;
+  mutable void* scratch = nullptr;  // reductions of the diagnostics, kept between calls
+  mutable size_t scratch_bytes = 0;
   hipEvent_t batch_done = nullptr;
 };
 
@@ -661,6 +663,7 @@ kg_status kg_destroy(kg_handle* h) {
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->batch_dev) (void)hipFree(h->batch_dev);
+  if (h->scratch) (void)hipFree(h->scratch);
   if (h->batch_done) (void)hipEventDestroy(h->batch_done);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -900,8 +903,25 @@ kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t*
 }  // extern "C"
 
 namespace kg {
+hipError_t handle_scratch(const kg_handle* h, size_t bytes, void** out) {
+  if (h->scratch_bytes < bytes) {
+    if (h->scratch) {
+      (void)hipStreamSynchronize(h->stream);
+      (void)hipFree(h->scratch);
+      h->scratch = nullptr;
+      h->scratch_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->scratch, bytes);
+    if (e != hipSuccess) return e;
+    h->scratch_bytes = bytes;
+  }
+  *out = h->scratch;
+  return hipSuccess;
+}
+
 int handle_view(const kg_handle* h, HandleView* v) {
   if (!h || !v) return 1;
+  v->handle = h;
   v->device = h->device;
   v->stream = h->stream;
   v->G = h->G;

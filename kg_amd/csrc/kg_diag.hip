@@ -195,7 +195,7 @@ hipError_t diag_integrate(const HandleView& v, int quantity, double out[2]) {
   const IntArgs a = make_args(v, quantity);
   const int nrows = (a.Ny + 1) * (a.Nz + 1);
   double* buf = nullptr;
-  hipError_t e = hipMalloc((void**)&buf, sizeof(double) * (2 * (size_t)nrows + 2));
+  hipError_t e = handle_scratch(v.handle, sizeof(double) * (2 * (size_t)nrows + 2), (void**)&buf);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(integrate_rows, dim3(nrows), dim3(256), 0, v.stream, a, buf + 2);
   e = hipGetLastError();
@@ -206,7 +206,6 @@ hipError_t diag_integrate(const HandleView& v, int quantity, double out[2]) {
   }
   if (e == hipSuccess) e = hipMemcpyAsync(out, buf, 2 * sizeof(double), hipMemcpyDeviceToHost, v.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
-  (void)hipFree(buf);
   return e;
 }
 }  // namespace kg

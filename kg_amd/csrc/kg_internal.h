@@ -139,6 +139,7 @@ hipError_t launch_pack(const Geom& G, double* dense, double* const fields[5], in
 // What the spectral projector needs from a handle: its device, stream, grid
 // and the current state's field buffers (these move between steps for RK4).
 struct HandleView {
+  const ::kg_handle* handle;
   int device;
   hipStream_t stream;
   Geom G;
@@ -150,6 +151,9 @@ struct HandleView {
   int metric;         // KG_METRIC_*
 };
 int handle_view(const ::kg_handle* h, HandleView* v);
+// Device scratch owned by the handle (grown on demand, freed by kg_destroy);
+// work on the handle's stream only.
+hipError_t handle_scratch(const ::kg_handle* h, size_t bytes, void** out);
 
 // Volume integral (integral.f90:646-669) of an integrand of the state, on the
 // view's stream, synchronous.  quantity 0..4: that field; 5: density*sqrtg;
