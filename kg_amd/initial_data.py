@@ -143,7 +143,7 @@ def shell_state(Nx, Ny, Nz, g, xmin, xmax, r0, width, amp, coeffs):
     return u
 
 
-def config1_state(Nx=256, Ny=33, Nz=65, g=2, xmin=1.8, xmax=60.0):
+def small_workload_state(Nx=256, Ny=33, Nz=65, g=2, xmin=1.8, xmax=60.0):
     """phi = 1e-3 exp(-(r-15)^2/1.5^2) Y_2^0, Pi = 0 (the small CPU-runnable workload)."""
     return shell_state(Nx, Ny, Nz, g, xmin, xmax, 15.0, 1.5, 1e-3, {(2, 0): 1.0})
 
@@ -153,7 +153,7 @@ def random_lm_coeffs(seed, lmax=4, std=1e-3):
     return {(l, m): float(rng.normal(0.0, std)) for l in range(lmax + 1) for m in range(-l, l + 1)}
 
 
-def config2_state(seed=20261203, Nx=2048, Ny=129, Nz=257, g=2, xmin=1.8, xmax=200.0):
+def full_workload_state(seed=20261203, Nx=2048, Ny=129, Nz=257, g=2, xmin=1.8, xmax=200.0):
     """phi = exp(-(r-30)^2/3^2) sum_{l<=4} c_lm Y_lm, c_lm ~ N(0, 1e-3) (the full single-GPU workload)."""
     return shell_state(Nx, Ny, Nz, g, xmin, xmax, 30.0, 3.0, 1.0, random_lm_coeffs(seed))
 

@@ -170,11 +170,11 @@ def test_fd4_small_reference_workload_2000_steps():
     massless, Kerr-Schild M=1, RK4 at CFL 0.25, 4th-order stencils, 2000 steps,
     phi = 1e-3 exp(-(r-15)^2/1.5^2) Y_2^0, Pi = 0): bit-identical to the oracle,
     so the per-field relative error is 0 against the 1e-12 bar."""
-    from kg_amd.initial_data import config1_state
+    from kg_amd.initial_data import small_workload_state
     print("fd4 small workload, 2000 steps", flush=True)
     sim = oracle.Sim(256, 33, 65, 2, 1.8, 60.0, "Kerr_Schild", 0.0, 1.0, 0.0, "outflow", 0.25)
     sim.set_order(4)
-    u0 = sim.fill(config1_state())
+    u0 = sim.fill(small_workload_state())
     sim.u[...] = u0
     gpu = LinesMethod(Nx=256, Ny=33, Nz=65, xmin=1.8, xmax=60.0, CFL=0.25, scheme="rk4", operators="fd4")
     gpu.set_u_soa(u0[None])

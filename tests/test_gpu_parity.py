@@ -123,11 +123,11 @@ def test_complex_state_is_two_real_systems():
     assert np.array_equal(got.imag, out[1])
 
 
-def test_config1_size_pirk2_and_rk4():
+def test_small_workload_size_pirk2_and_rk4():
     """The small reference workload (Nr=256, Ntheta=32, Nphi=64, r in [1.8, 60]) for a few steps."""
     import kg_amd
-    from kg_amd.initial_data import config1_state
-    u0 = config1_state()
+    from kg_amd.initial_data import small_workload_state
+    u0 = small_workload_state()
     for scheme, n in (("pirk2", 6), ("rk4", 3)):
         sim = oracle.Sim(256, 33, 65, 2, 1.8, 60.0, "Kerr_Schild", 0.0, 1.0, 0.0, "outflow", 0.25)
         sim.u[...] = u0
