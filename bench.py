@@ -114,23 +114,23 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     (b) one at a time through the plain path (kg_step graph replay per grid).
     Aggregate grid-point updates/s of each; inputs resident in HBM."""
     import torch
+    from kg_amd.autotune import autotune_batch
     from kg_amd.batch import batch_step, make_member, uneven_specs
 
     specs = specs if specs is not None else uneven_specs(n)
     members = []
     for s in specs:
         m = make_member(s, on_device=True, operators=operators)
-        m.set_tuning(*tuning)
         members.append(m)
     torch.cuda.empty_cache()
     pts = sum(s.points for s in specs)
-    batch_step(members, 1)  # warm-up
-    members[0].sync()
+    batch_kc, _ = autotune_batch(members, tj=tuning[1])  # includes the warm-up
     t0 = time.perf_counter()
     batch_step(members, steps)
     members[0].sync()
     tb = time.perf_counter() - t0
-    for m in members:  # warm-up (graph capture) of the plain path
+    for m in members:  # the plain path's shape for these grids; warm-up (graph capture)
+        m.set_tuning(*tuning)
         m.step(1)
         m.sync()
     tp = 0.0
@@ -143,7 +143,8 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     for m in members:
         m.close()
     vb, vp = pts * steps / tb, pts * steps / tp
-    return {"members": n, "steps": steps, "operators": operators, "grid_points": pts, "batched_updates_per_s": vb,
+    return {"members": n, "steps": steps, "operators": operators, "batch_kc": batch_kc,
+            "plain_tuning": {"kc": tuning[0], "tj": tuning[1]}, "grid_points": pts, "batched_updates_per_s": vb,
             "plain_updates_per_s": vp, "speedup": vb / vp,
             "workload": "uneven RK4 batch: Nr~U{256..2048}, (Ntheta,Nphi) in {(16,32),(32,64),(64,128)}, "
                         "m~U[0,0.5], M~U[0.5,2], seed 20261203",
@@ -203,7 +204,7 @@ def initial_data(ncart, reps=3):
             "amp": amp, "state": "complex, 5 fields on the full grid"}
 
 
-def serving(nreq, tuning, operators):
+def serving(nreq, tuning, operators, batch_kc=None):
     """Serving loop: nreq requests of the batched workload arriving as a Poisson
     stream (50/s), steps ~U{500..5000}, admitted against the free HBM and run by
     continuous batching; each returns its final fields and its spectrum.  The
@@ -212,7 +213,7 @@ def serving(nreq, tuning, operators):
     from kg_amd.serving import GpuEngine, ServingLoop, percentile, poisson_requests
 
     reqs = poisson_requests(nreq)
-    eng = GpuEngine(tuning=tuning, operators=operators)
+    eng = GpuEngine(tuning=tuning, operators=operators, batch_kc=batch_kc)
     free, _ = torch.cuda.mem_get_info()
     budget = int(0.85 * free)
     results, stats = ServingLoop(eng, budget, chunk=50).run(reqs, keep_fields=True)
@@ -223,7 +224,8 @@ def serving(nreq, tuning, operators):
         eng.run_alone(r, keep_fields=True)
     tp = time.perf_counter() - t0
     ups_plain = stats.updates / tp
-    return {"requests": nreq, "operators": operators, "arrival_rate_per_s": 50.0, "hbm_budget_bytes": budget,
+    return {"requests": nreq, "operators": operators, "batch_kc": batch_kc, "plain_tuning": list(tuning),
+            "arrival_rate_per_s": 50.0, "hbm_budget_bytes": budget,
             "p50_time_to_result_s": percentile(lat, 50), "p99_time_to_result_s": percentile(lat, 99),
             "served_updates_per_s": stats.updates_per_s, "one_at_a_time_updates_per_s": ups_plain,
             "speedup": stats.updates_per_s / ups_plain, "max_active": stats.max_active,
@@ -374,9 +376,10 @@ def main():
     if args.ic > 0:
         print("bench: initial data, ncart %d" % args.ic, file=sys.stderr, flush=True)
         out["initial_data"] = initial_data(args.ic)
-    # the batched and serving legs run the timed workload's operator set; their
-    # grids are far smaller than the plain one, so kc = 8 for every member
-    shape = (8, 8) if args.operators == "fd4" else tuple(head["tuning"].values())
+    # the batched and serving legs run the timed workload's operator set; a
+    # member run alone uses kc = 8 (its grid is far smaller than the plain one),
+    # batched members the phi chunk the batch autotuner picks
+    shape = (8, 8)
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
         out["batched"] = batched_vs_plain(args.batched, args.batched_steps, shape, args.operators)
@@ -389,7 +392,8 @@ def main():
         out["batched_small_members"] = small
     if args.serve > 0:
         print("bench: serving loop, %d requests" % args.serve, file=sys.stderr, flush=True)
-        out["serving"] = serving(args.serve, shape, args.operators)
+        out["serving"] = serving(args.serve, shape, args.operators,
+                                 out["batched"]["batch_kc"] if "batched" in out else None)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.operators)

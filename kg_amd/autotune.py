@@ -11,6 +11,8 @@ saved before and restored after.
 """
 from __future__ import annotations
 
+import time
+
 import numpy as np
 
 CANDIDATES = [(kc, tj) for tj in (4, 8) for kc in (8, 16, 32, 64)]
@@ -44,4 +46,27 @@ def autotune(lm, candidates=CANDIDATES, steps=2, verbose=False):
     if aux is not None:
         for w, arr in enumerate(aux):
             lm.set_u_stage_soa(w, arr)
+    return best, timings
+
+
+def autotune_batch(members, candidates=(8, 16, 32), steps=2, tj=8):
+    """Pick one phi-chunk length for a batch (every member gets it) by timing
+    `steps` batched steps per candidate after one warm-up step.  The members
+    advance (1 + steps) steps per candidate: tune before the work whose state
+    matters, or on copies.  Returns (kc, {kc: seconds per batched step})."""
+    from .batch import batch_step
+
+    timings = {}
+    for kc in candidates:
+        for m in members:
+            m.set_tuning(kc, tj)
+        batch_step(members, 1)
+        members[0].sync()
+        t0 = time.perf_counter()
+        batch_step(members, steps)
+        members[0].sync()
+        timings[kc] = (time.perf_counter() - t0) / steps
+    best = min(timings, key=timings.get)
+    for m in members:
+        m.set_tuning(best, tj)
     return best, timings

@@ -459,7 +459,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
                        roles(&P, nullptr, &h->Ub, nullptr, &h->Ua), roles(&P, &P, &X, &X, &h->Ub, &h->Ua)};
       for (int st = 0; st < nst; ++st) {
         StageArgs a = make_stage_args(h, modes[st], fb, rl[st]);
-        a.kc = lead->tune_kc;
+        a.kc = h->tune_kc;  // each member keeps its own phi chunk
         a.tj = lead->G.fd4 ? 8 : lead->tune_tj;  // the fd4 stages run 8 theta rows per workgroup
         finalize_tiles(a);
         sd[(v * nst + st) * n + q] = a;
@@ -864,6 +864,7 @@ kg_status kg_sync(kg_handle* h) {
 kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj) {
   if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
   if (kc < 1 || kc > 4096 || (tj != 4 && tj != 8)) return fail(h, KG_ERR_INVALID, "tuning: kc in [1,4096], tj in {4,8}");
+  if (kc == h->tune_kc && tj == h->tune_tj) return KG_OK;
   KG_HIP(h, hipStreamSynchronize(h->stream));
   h->tune_kc = kc;
   h->tune_tj = tj;

@@ -172,9 +172,12 @@ class GpuEngine:
     projector; ``step`` is one batched launch per RK4 stage."""
 
     def __init__(self, lmax: int = 8, nk: int = 512, kmax: float = 4.0, CFL: float = 0.25, tuning=None,
-                 on_device: bool = True, operators: str = "ref2"):
+                 on_device: bool = True, operators: str = "ref2", batch_kc=None):
+        """tuning: (kc, tj) of a request run alone; batch_kc: the phi chunk of
+        requests advanced together (defaults to tuning's)."""
         self.lmax, self.nk, self.kmax, self.CFL = lmax, nk, kmax, CFL
         self.tuning = tuning
+        self.batch_kc = batch_kc
         self.on_device = on_device
         self.operators = operators
 
@@ -204,6 +207,10 @@ class GpuEngine:
     def step(self, members, nsteps: int):
         from .batch import batch_step
 
+        if self.batch_kc:
+            tj = self.tuning[1] if self.tuning else 8
+            for m in members:
+                m[0].set_tuning(self.batch_kc, tj)  # a no-op when unchanged
         batch_step([m[0] for m in members], nsteps)
 
     def finish(self, member, keep_fields: bool = True):
