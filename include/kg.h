@@ -146,16 +146,18 @@ kg_status kg_step(kg_handle* h, int32_t nsteps);
  * per process).  Advances every handle of hs[0..n) by nsteps RK4 steps, each
  * stage one kernel launch over the tiles of all grids (grids may differ in
  * size, mass, black-hole parameters and initial data; all must be RK4 with
- * the same metric form and operator set).  Work runs on hs[0]'s stream; later calls on any
- * member are ordered after it.  Each member's result equals stepping it
- * alone. */
+ * the same metric form and operator set).  Each member marches its own phi
+ * chunk (kg_set_tuning kc); theta rows per workgroup are hs[0]'s.  Work runs
+ * on hs[0]'s stream; later calls on any member are ordered after it.  Each
+ * member's result equals stepping it alone. */
 kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
 /* waits for the handle's stream */
 kg_status kg_sync(kg_handle* h);
 
 /* Launch shape of the stage kernel (no reference counterpart): kc = phi
- * planes each workgroup marches over, tj = theta rows per workgroup (4 or 8).
- * Results do not depend on it; kg_autotune-style callers sweep it. */
+ * planes each workgroup marches over, tj = theta rows per workgroup (4 or 8;
+ * the fd4 stages always run 8).  Results do not depend on it; autotuning
+ * callers sweep it.  Setting the current shape again is a no-op. */
 kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj);
 
 /* Kernel timing with HIP events on the handle's stream (no reference
