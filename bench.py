@@ -224,7 +224,8 @@ def serving(nreq, tuning, operators, batch_kc=None):
         eng.run_alone(r, keep_fields=True)
     tp = time.perf_counter() - t0
     ups_plain = stats.updates / tp
-    return {"requests": nreq, "operators": operators, "batch_kc": batch_kc, "plain_tuning": list(tuning),
+    return {"requests": nreq, "operators": operators, "batch_kc": batch_kc,
+            "kc_chunks": {str(k): v for k, v in sorted(eng.kc_chunks.items())}, "plain_tuning": list(tuning),
             "arrival_rate_per_s": 50.0, "hbm_budget_bytes": budget,
             "p50_time_to_result_s": percentile(lat, 50), "p99_time_to_result_s": percentile(lat, 99),
             "served_updates_per_s": stats.updates_per_s, "one_at_a_time_updates_per_s": ups_plain,
@@ -378,7 +379,8 @@ def main():
         out["initial_data"] = initial_data(args.ic)
     # the batched and serving legs run the timed workload's operator set; a
     # member run alone uses kc = 8 (its grid is far smaller than the plain one),
-    # batched members the phi chunk the batch autotuner picks
+    # batched members the phi chunk the batch autotuner picks, served ones the
+    # chunk the workgroup-fill rule gives for the requests active in each chunk
     shape = (8, 8)
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)
@@ -392,8 +394,7 @@ def main():
         out["batched_small_members"] = small
     if args.serve > 0:
         print("bench: serving loop, %d requests" % args.serve, file=sys.stderr, flush=True)
-        out["serving"] = serving(args.serve, shape, args.operators,
-                                 out["batched"]["batch_kc"] if "batched" in out else None)
+        out["serving"] = serving(args.serve, shape, args.operators, "fill")
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.operators)

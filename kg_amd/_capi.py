@@ -119,6 +119,16 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("kg_amd/libkg_hip.so is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    # One HIP runtime per process.  torch's ROCm wheel ships its own
+    # libamdhip64 (soname libamdhip64.so.7), which its libraries request as
+    # "libamdhip64.so".  Loaded first, libkg_hip.so would bind ROCm's
+    # libamdhip64.so.7 and torch would later load a second runtime that sees no
+    # GPU; with torch imported first, libkg_hip.so binds the runtime torch
+    # loaded (same soname), and device pointers and streams are shared.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -70,3 +70,29 @@ def autotune_batch(members, candidates=(8, 16, 32), steps=2, tj=8):
     for m in members:
         m.set_tuning(best, tj)
     return best, timings
+
+
+
+def stage_tile_planes(lm, tj=8):
+    """Phi planes marched by all workgroups of one RK stage of LinesMethod
+    `lm`, summed: the stages of a step followed by a ghost fill compute
+    i in [1, ni-2], j in [g+1, g+Ny-1] and every phi plane, in tiles of 64
+    r-points by `tj` theta rows, once per real system."""
+    nsys = 2 if lm.is_complex else 1
+    return (-(-(lm.ni - 2) // 64)) * (-(-(lm.Ny - 1) // tj)) * lm.Nz * nsys
+
+
+def fill_kc(members, candidates=(8, 16, 32), tj=8, min_workgroups=4096):
+    """The longest phi chunk that still leaves `min_workgroups` full-length
+    chunks of work per stage over the batch (16 per CU), else the shortest
+    candidate.  Longer chunks re-read fewer phi edges; shorter ones keep the
+    CUs busy when few or small grids are active.  On one MI355X the rule gives
+    the timed picks of `autotune` / `autotune_batch`: 32 for one 2048 x 128 x 256
+    grid and for the 256-member uneven population, 16 for 256 grids of
+    256 x 16 x 32."""
+    work = sum(stage_tile_planes(m, tj) for m in members)
+    cands = sorted(candidates)
+    for kc in reversed(cands):
+        if work >= kc * min_workgroups:
+            return kc
+    return cands[0]

@@ -26,15 +26,29 @@
 namespace kg {
 namespace {
 
+// Every device buffer the kernels touch is hipMalloc'd global memory, and is
+// accessed through address-space-1 pointers.  A pointer read from a
+// descriptor table in memory (the batched kernels) is otherwise generic and
+// compiles to flat loads and stores, which also count against the LDS
+// counter: each wait for a field then drains the tile's LDS traffic as well
+// (measured 7-15 % on a batched stage launch).
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) char gchar;
+typedef __attribute__((address_space(1))) const char gcchar;
+
+__device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
+__device__ __forceinline__ gcdouble* gptr(const double* p) { return (gcdouble*)p; }
+
 // Global access as (uniform 64-bit base) + (32-bit byte offset): the base
 // stays in SGPRs and a lane carries one 32-bit offset, which maps to the
 // SGPR-base form of global_load/global_store.  Every field of one system is
 // below 2 GiB (checked at kg_create), so offsets fit.
 __device__ __forceinline__ double gld(const double* base, uint32_t off) {
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+  return *(gcdouble*)((gcchar*)base + off);
 }
 __device__ __forceinline__ void gst(double* base, uint32_t off, double v) {
-  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off) = v;
+  *(gdouble*)((gchar*)base + off) = v;
 }
 
 constexpr int TI = 64;  // tile width along r: one wavefront per theta row
@@ -1099,10 +1113,10 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
     r /= G.nk;
     const int c = (int)(r % 5);
     const int64_t sys = (r / 5) * G.sys_stride;
-    double* u = a.u[c] + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
+    gdouble* u = gptr(a.u[c]) + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
     u[0] = 3.0 * u[1] - 3.0 * u[2] + u[3];
     if (G.bc_inflow) {
-      const double* up = a.u_p[c] + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
+      gcdouble* up = gptr(a.u_p[c]) + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
       u[G.ni - 1] = up[G.ni - 1];
     } else {
       u[G.ni - 1] = u[G.ni - 2];
@@ -1129,7 +1143,7 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
       jd = G.g + G.Ny + gg;       // j = Ny+gg
       js = G.g + G.Ny - gg - 1;   // j = Ny-gg-1
     }
-    double* u = a.u[c] + sys;
+    gdouble* u = gptr(a.u[c]) + sys;
     u[(int64_t)kk * G.plane + (int64_t)jd * G.ld + ii] = u[(int64_t)kt * G.plane + (int64_t)js * G.ld + ii];
   } else {
     // boundary_zL (:587) and boundary_zR (:596)
@@ -1139,7 +1153,7 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
     r /= 2;
     const int c = (int)(r % 5);
     const int64_t sys = (r / 5) * G.sys_stride;
-    double* u = a.u[c] + sys;
+    gdouble* u = gptr(a.u[c]) + sys;
     if (side == 0)
       u[t] = u[(int64_t)(G.Nz - 1) * G.plane + t];
     else
@@ -1177,15 +1191,16 @@ __global__ __launch_bounds__(256) void pack_kernel(const Geom G, double* __restr
   if (ii >= G.ni) return;
   const int c = blockIdx.z % 5;
   const int s = blockIdx.z / 5;
-  double* dst[5] = {d0, d1, d2, d3, d4};
+  gdouble* fld = gptr(c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : c == 3 ? d3 : d4);
+  gdouble* dns = gptr(src);
   const int64_t p = ((int64_t)kk * G.nj + jj) * G.ni + ii;
   const int64_t dense =
       fortran ? (p * 5 + c) * nsys + s : ((((int64_t)s * 5 + c) * G.nk + kk) * G.nj + jj) * G.ni + ii;
   const int64_t pitched = (int64_t)s * G.sys_stride + (int64_t)kk * G.plane + (int64_t)jj * G.ld + ii;
   if (to_dev)
-    dst[c][pitched] = src[dense];
+    fld[pitched] = dns[dense];
   else
-    src[dense] = dst[c][pitched];
+    dns[dense] = fld[pitched];
 }
 
 template <int MODE>

@@ -172,12 +172,16 @@ class GpuEngine:
     projector; ``step`` is one batched launch per RK4 stage."""
 
     def __init__(self, lmax: int = 8, nk: int = 512, kmax: float = 4.0, CFL: float = 0.25, tuning=None,
-                 on_device: bool = True, operators: str = "ref2", batch_kc=None):
+                 on_device: bool = True, operators: str = "ref2", batch_kc=None, fill_workgroups: int = 4096):
         """tuning: (kc, tj) of a request run alone; batch_kc: the phi chunk of
-        requests advanced together (defaults to tuning's)."""
+        requests advanced together (defaults to tuning's), or "fill" to pick it
+        per chunk from the active set (``autotune.fill_kc`` with
+        ``fill_workgroups`` full-length chunks per stage)."""
         self.lmax, self.nk, self.kmax, self.CFL = lmax, nk, kmax, CFL
         self.tuning = tuning
         self.batch_kc = batch_kc
+        self.fill_workgroups = int(fill_workgroups)
+        self.kc_chunks = {}  # phi chunk -> batched calls that used it
         self.on_device = on_device
         self.operators = operators
 
@@ -209,8 +213,14 @@ class GpuEngine:
 
         if self.batch_kc:
             tj = self.tuning[1] if self.tuning else 8
+            kc = self.batch_kc
+            if kc == "fill":
+                from .autotune import fill_kc
+
+                kc = fill_kc([m[0] for m in members], tj=tj, min_workgroups=self.fill_workgroups)
+            self.kc_chunks[kc] = self.kc_chunks.get(kc, 0) + 1
             for m in members:
-                m[0].set_tuning(self.batch_kc, tj)  # a no-op when unchanged
+                m[0].set_tuning(kc, tj)  # a no-op when unchanged
         batch_step([m[0] for m in members], nsteps)
 
     def finish(self, member, keep_fields: bool = True):

@@ -41,3 +41,22 @@ def test_served_results_equal_running_alone():
         assert np.array_equal(res.fields, fields), req.rid
         assert np.array_equal(res.spectrum, spectrum), req.rid
         assert np.isfinite(spectrum).all() and np.abs(spectrum).max() > 0
+
+
+def test_fd4_served_with_per_chunk_phi_chunks_equal_running_alone():
+    """fd4 stencils, the phi chunk chosen per batched call from the active set
+    (a tiny workgroup target, so the choice changes as requests come and go):
+    still bit-identical to each request run alone."""
+    eng = GpuEngine(lmax=4, nk=16, kmax=4.0, on_device=False, operators="fd4", tuning=(8, 8), batch_kc="fill",
+                    fill_workgroups=4)
+    reqs = _requests()
+    sizes = sorted(eng.bytes_for(r.spec) for r in reqs)
+    loop = ServingLoop(eng, sizes[-1] + sizes[-2] + sizes[-3], chunk=4)
+    print("serving %d fd4 requests, phi chunk per call" % len(reqs), flush=True)
+    results, stats = loop.run(reqs)
+    assert len(eng.kc_chunks) >= 2, eng.kc_chunks
+    for req, res in zip(reqs, results):
+        assert not res.error
+        fields, spectrum = eng.run_alone(req)
+        assert np.array_equal(res.fields, fields), req.rid
+        assert np.array_equal(res.spectrum, spectrum), req.rid

@@ -131,3 +131,24 @@ def test_poisson_stream_and_percentiles():
     assert poisson_requests(10, seed=5)[3].nsteps == poisson_requests(10, seed=5)[3].nsteps
     assert percentile([1, 2, 3, 4], 50) == pytest.approx(2.5)
     assert percentile(list(range(101)), 99) == pytest.approx(99.0)
+
+
+def test_fill_rule_phi_chunks():
+    """The workgroup-fill rule: the longest phi chunk leaving 4096 full-length
+    chunks per stage; measured picks it reproduces (one 2048x128x256 grid: 32;
+    256 grids of 256x16x32, 33 phi planes each: 16) and its fallback."""
+    from types import SimpleNamespace
+
+    from kg_amd.autotune import fill_kc, stage_tile_planes
+
+    def grid(Nr, Nt, Np, cx=False):
+        return SimpleNamespace(ni=Nr + 5, Ny=Nt + 1, Nz=Np + 1, is_complex=cx)
+
+    big = grid(2048, 128, 256)
+    assert stage_tile_planes(big) == 33 * 16 * 257
+    assert stage_tile_planes(grid(2048, 128, 256, True)) == 2 * 33 * 16 * 257
+    assert fill_kc([big]) == 32
+    assert fill_kc([grid(256, 16, 32)] * 256) == 16
+    assert fill_kc([grid(256, 16, 32)]) == 8  # nothing fills: the shortest
+    assert fill_kc([big] * 4, candidates=(8, 16, 32, 64)) == 64
+    assert fill_kc([grid(24, 8, 16)], min_workgroups=1) == 16  # 17 planes: a 32 chunk would leave < 1
