@@ -37,6 +37,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 B_PER_UPDATE = {"rk4": 680, "rk3": 320, "pirk2": 248, "pirk3": 416}
+# phi chunks the plain path's autotuner chooses from for a grid run by itself
+# (a grid of 256 x 16 x 32 is fastest at 2, one of 1024 x 16 x 32 and larger at 8)
+PLAIN_KC = (1, 2, 4, 8, 16)
 STAGES = {"rk4": 4, "rk3": 3, "pirk2": 3, "pirk3": 5}
 
 CFG = dict(Nx=2048, Ny=129, Nz=257, xmin=1.8, xmax=200.0, m=0.1, M_bh=1.0, a=0.0, CFL=0.25, scheme="rk4")
@@ -111,10 +114,11 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     """The uneven batched workload: n evolutions (Nr in [256, 2048], (Ntheta, Nphi)
     in {(16,32), (32,64), (64,128)}, m ~ U[0,0.5], M ~ U[0.5,2], RK4), advanced
     `steps` steps (a) all together, one launch per stage (kg_batch_step), and
-    (b) one at a time through the plain path (kg_step graph replay per grid).
+    (b) one at a time through the plain path (kg_step graph replay per grid,
+    each grid at the phi chunk its own autotuner picks among PLAIN_KC).
     Aggregate grid-point updates/s of each; inputs resident in HBM."""
     import torch
-    from kg_amd.autotune import autotune_batch
+    from kg_amd.autotune import autotune, autotune_batch
     from kg_amd.batch import batch_step, make_member, uneven_specs
 
     specs = specs if specs is not None else uneven_specs(n)
@@ -129,8 +133,12 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     batch_step(members, steps)
     members[0].sync()
     tb = time.perf_counter() - t0
-    for m in members:  # the plain path's shape for these grids; warm-up (graph capture)
-        m.set_tuning(*tuning)
+    # the plain path at its own best shape for each grid (its autotuner, outside
+    # the timed region), then a warm-up step (graph capture)
+    picks = {}
+    for m in members:
+        best, _ = autotune(m, candidates=[(kc, tuning[1]) for kc in PLAIN_KC], steps=2)
+        picks[best[0]] = picks.get(best[0], 0) + 1
         m.step(1)
         m.sync()
     tp = 0.0
@@ -144,7 +152,8 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
         m.close()
     vb, vp = pts * steps / tb, pts * steps / tp
     return {"members": n, "steps": steps, "operators": operators, "batch_kc": batch_kc,
-            "plain_tuning": {"kc": tuning[0], "tj": tuning[1]}, "grid_points": pts, "batched_updates_per_s": vb,
+            "plain_tuning": "autotuned per grid", "plain_kc_picks": {str(k): v for k, v in sorted(picks.items())},
+            "grid_points": pts, "batched_updates_per_s": vb,
             "plain_updates_per_s": vp, "speedup": vb / vp,
             "workload": "uneven RK4 batch: Nr~U{256..2048}, (Ntheta,Nphi) in {(16,32),(32,64),(64,128)}, "
                         "m~U[0,0.5], M~U[0.5,2], seed 20261203",
@@ -210,6 +219,7 @@ def serving(nreq, tuning, operators, batch_kc=None):
     continuous batching; each returns its final fields and its spectrum.  The
     same requests one at a time through the plain path give the comparison."""
     import torch
+    from kg_amd.autotune import autotune
     from kg_amd.serving import GpuEngine, ServingLoop, percentile, poisson_requests
 
     reqs = poisson_requests(nreq)
@@ -219,13 +229,21 @@ def serving(nreq, tuning, operators, batch_kc=None):
     results, stats = ServingLoop(eng, budget, chunk=50).run(reqs, keep_fields=True)
     lat = [r.latency for r in results]
     del results
+    # each request alone at the plain path's best shape for its grid (tuned
+    # outside the timed region)
+    alone = {}
+    for r in reqs:
+        member = eng.admit(r)
+        alone[r.rid], _ = autotune(member[0], candidates=[(kc, tuning[1]) for kc in PLAIN_KC], steps=2)
+        eng.release(member)
     t0 = time.perf_counter()
     for r in reqs:
-        eng.run_alone(r, keep_fields=True)
+        eng.run_alone(r, keep_fields=True, tuning=alone[r.rid])
     tp = time.perf_counter() - t0
     ups_plain = stats.updates / tp
     return {"requests": nreq, "operators": operators, "batch_kc": batch_kc,
-            "kc_chunks": {str(k): v for k, v in sorted(eng.kc_chunks.items())}, "plain_tuning": list(tuning),
+            "kc_chunks": {str(k): v for k, v in sorted(eng.kc_chunks.items())},
+            "alone_tuning": {str(r.rid): list(alone[r.rid]) for r in reqs},
             "arrival_rate_per_s": 50.0, "hbm_budget_bytes": budget,
             "p50_time_to_result_s": percentile(lat, 50), "p99_time_to_result_s": percentile(lat, 99),
             "served_updates_per_s": stats.updates_per_s, "one_at_a_time_updates_per_s": ups_plain,
@@ -378,9 +396,10 @@ def main():
         print("bench: initial data, ncart %d" % args.ic, file=sys.stderr, flush=True)
         out["initial_data"] = initial_data(args.ic)
     # the batched and serving legs run the timed workload's operator set; a
-    # member run alone uses kc = 8 (its grid is far smaller than the plain one),
-    # batched members the phi chunk the batch autotuner picks, served ones the
-    # chunk the workgroup-fill rule gives for the requests active in each chunk
+    # grid run alone uses the phi chunk its own autotuner picks (PLAIN_KC),
+    # batched members the chunk the batch autotuner picks, served ones the
+    # chunk the workgroup-fill rule gives for the requests active in each call;
+    # theta rows per workgroup are 8 throughout
     shape = (8, 8)
     if args.batched > 0:
         print("bench: batched vs plain, %d members" % args.batched, file=sys.stderr, flush=True)

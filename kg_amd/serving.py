@@ -234,11 +234,13 @@ class GpuEngine:
         sp.close()
         lm.close()
 
-    def run_alone(self, req: Request, keep_fields: bool = True):
+    def run_alone(self, req: Request, keep_fields: bool = True, tuning=None):
         """The same request through the plain path: its own grid, kg_step graph
-        replay, then the same read-back."""
+        replay (at `tuning` = (kc, tj) when given), then the same read-back."""
         member = self.admit(req)
         try:
+            if tuning:
+                member[0].set_tuning(*tuning)
             member[0].step(req.nsteps)
             return self.finish(member, keep_fields)
         finally:
