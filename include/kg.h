@@ -143,13 +143,16 @@ kg_status kg_rk4(kg_handle* h);
  * without diagnostics, replayed from a captured HIP graph. */
 kg_status kg_step(kg_handle* h, int32_t nsteps);
 /* Batched stepping (no reference counterpart: the reference runs one grid
- * per process).  Advances every handle of hs[0..n) by nsteps RK4 steps, each
- * stage one kernel launch over the tiles of all grids (grids may differ in
- * size, mass, black-hole parameters and initial data; all must be RK4 with
- * the same metric form and operator set).  Each member marches its own phi
- * chunk (kg_set_tuning kc); theta rows per workgroup are hs[0]'s.  Work runs
- * on hs[0]'s stream; later calls on any member are ordered after it.  Each
- * member's result equals stepping it alone. */
+ * per process).  Advances every handle of hs[0..n) by nsteps steps of their
+ * common desc.scheme (RK4, RK3, PIRK2 or PIRK3), each stage kernel and each
+ * ghost-fill pass one launch over the tiles of all grids (grids may differ in
+ * size, real or complex state, mass, black-hole parameters, initial data and
+ * buffer or stale-Pi parity; all must share the scheme, the metric form and
+ * the operator set).  Each member marches its own phi chunk (kg_set_tuning
+ * kc); theta rows per workgroup are hs[0]'s.  Work runs on hs[0]'s stream;
+ * later calls on any member are ordered after it.  Each member's result,
+ * including its PIRK stage arrays, equals stepping it alone.
+ * KG_ERR_UNSUPPORTED for mixed schemes, operator sets or metric forms. */
 kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
 /* waits for the handle's stream */
 kg_status kg_sync(kg_handle* h);

@@ -28,7 +28,9 @@ DR = 58.2 / 256  # radial spacing of Nr = 256 on r in [1.8, 60]
 
 
 def batch_step(members, nsteps: int = 1) -> None:
-    """Advance every LinesMethod in `members` by nsteps RK4 steps in batched launches."""
+    """Advance every LinesMethod in `members` by nsteps steps of their common
+    scheme (rk4, rk3, pirk2 or pirk3) in batched launches: one per stage
+    kernel and one per ghost-fill pass over all members."""
     if not members:
         return
     lib = capi.load()
@@ -72,20 +74,26 @@ def uneven_specs(n: int = 256, seed: int = 20261203):
     return out
 
 
-def make_member(spec: MemberSpec, CFL: float = 0.25, on_device: bool = False, operators: str = "ref2") -> LinesMethod:
+def make_member(spec: MemberSpec, CFL: float = 0.25, on_device: bool = False, operators: str = "ref2",
+                scheme: str = "rk4", is_complex: bool = False) -> LinesMethod:
     """Create the grid on the GPU and upload its ghost-filled initial data
-    (built on the host, or with torch on the GPU when on_device)."""
+    (built on the host, or with torch on the GPU when on_device; a complex
+    member gets the shell as its real part and half of it as its imaginary
+    part)."""
     lm = LinesMethod(Nx=spec.Nr, Ny=spec.Ntheta + 1, Nz=spec.Nphi + 1, xmin=spec.xmin, xmax=spec.xmax, CFL=CFL,
-                     m=spec.m, M_bh=spec.M_bh, scheme="rk4", operators=operators)
+                     m=spec.m, M_bh=spec.M_bh, scheme=scheme, operators=operators, is_complex=is_complex)
     args = (spec.Nr, spec.Ntheta + 1, spec.Nphi + 1, 2, spec.xmin, spec.xmax, spec.r0, spec.width, 1.0, spec.coeffs)
     if on_device:
         import torch
         from .initial_data import shell_state_device
         u0 = shell_state_device(*args)
+        if is_complex:
+            u0 = torch.stack([u0, 0.5 * u0])
         torch.cuda.synchronize()
         lm.copy_state_device(u0.data_ptr(), to_handle=True)
         del u0
     else:
-        lm.set_u_soa(shell_state(*args)[None])
+        u0 = shell_state(*args)
+        lm.set_u_soa(np.stack([u0, 0.5 * u0]) if is_complex else u0[None])
     lm.boundary()
     return lm

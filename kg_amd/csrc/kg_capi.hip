@@ -36,6 +36,15 @@ struct State5 {
 
 }  // namespace
 
+// One launch of a step, recorded instead of launched (batched stepping
+// replays the recorded sequence of every member with one launch per entry).
+struct RecOp {
+  int kind;  // 0: stage kernel, 1: ghost fill (its three passes)
+  int mode;
+  StageArgs a;
+  FillArgs f;
+};
+
 struct kg_handle {
   kg_desc desc{};
   HostGrid hg{};
@@ -70,6 +79,7 @@ struct kg_handle {
   mutable void* scratch = nullptr;  // reductions of the diagnostics, kept between calls
   mutable size_t scratch_bytes = 0;
   hipEvent_t batch_done = nullptr;
+  std::vector<RecOp>* rec = nullptr;  // when set, stage() and fill() record instead of launching
 };
 
 namespace {
@@ -190,6 +200,10 @@ kg_status stage(kg_handle* h, int mode, const Box& b, const Roles& roles) {
   a.kc = h->tune_kc;
   a.tj = h->tune_tj;
   a.nsys = h->nsys;
+  if (h->rec) {
+    h->rec->push_back(RecOp{0, mode, a, FillArgs{}});
+    return KG_OK;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof) {
     e0 = next_event(h);
@@ -223,6 +237,10 @@ kg_status fill(kg_handle* h, const State5& u, const State5& u_p) {
   for (int c = 0; c < 5; ++c) {
     f.u[c] = u.f[c];
     f.u_p[c] = u_p.f[c];
+  }
+  if (h->rec) {
+    h->rec->push_back(RecOp{1, -1, StageArgs{}, f});
+    return KG_OK;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof) {
@@ -404,79 +422,79 @@ kg_status xfer(kg_handle* h, const State5& st, double* host, int32_t layout, int
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Batched RK4: every handle of a batch advances with one launch per stage.
+// Batched stepping: every handle of a batch advances with one launch per
+// stage kernel and one per fill pass.  A step of each member is recorded
+// (the launches enqueue_step would make, with the member's own buffers) for
+// both values of its buffer and stale-Pi parities, then replayed for all
+// members together: member q's part of launch o is exactly its own launch o.
 // ---------------------------------------------------------------------------
-StageArgs make_stage_args(kg_handle* h, int mode, const Box& b, const Roles& rl) {
-  StageArgs a{};
-  a.G = h->G;
-  a.tab = h->d_tab;
-  a.per = (int64_t)h->G.sys_stride * h->nsys;
-  for (int r = 0; r < R_NUM; ++r) {
-    const State5* st = rl.r[r];
-    a.rb[r] = st ? st->alloc : nullptr;
-    a.ri4[r] = st && st->f[4] ? (int)((st->f[4] - st->alloc) / a.per) : 4;
-  }
-  a.i_lo = b.i_lo;
-  a.i_hi = b.i_hi;
-  a.j_lo = b.j_lo;
-  a.j_hi = b.j_hi;
-  a.k_lo = b.k_lo;
-  a.k_hi = b.k_hi;
-  a.nsys = h->nsys;
-  return a;
-}
-
 kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
   kg_handle* lead = hs[0];
   const int ef = lead->G.metric_ef;
+  const int scheme = lead->desc.scheme;
   for (int q = 0; q < n; ++q) {
     kg_handle* h = hs[q];
     if (!h) return fail(lead, KG_ERR_INVALID, "batch: null handle %d", q);
-    if (h->desc.scheme != KG_RK4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d is not RK4", q);
+    if (h->desc.scheme != scheme) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d runs another scheme", q);
     if (h->G.fd4 != lead->G.fd4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed operator sets");
     if (h->G.metric_ef != ef) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed metric forms");
     if (h->device != lead->device) return fail(lead, KG_ERR_INVALID, "batch: handles on different devices");
     for (int p = 0; p < q; ++p)
       if (hs[p] == h) return fail(lead, KG_ERR_INVALID, "batch: handle %d repeated", q);
-    KG_TRY(ensure_rk(h, true));
+    KG_TRY(ensure_buffers(h, scheme));
     if (h != lead) KG_HIP(lead, hipStreamSynchronize(h->stream));
   }
-  // descriptors for both buffer parities: version v has u and the alternate
-  // buffer of every handle exchanged v times
-  const int nst = 4;
-  const int modes[nst] = {M_RK4_S1, M_RK4_S2, M_RK4_S3, M_RK4_S4};
-  std::vector<StageArgs> sd(2 * nst * n);
-  std::vector<FillArgs> fd(2 * n);
-  std::vector<int> tstart(n + 1), bstart(3 * (n + 1));
+  // two consecutive steps of every member: version v is the step taken after
+  // v parity advances (each parity has period 2 under advance_parity)
+  std::vector<std::vector<RecOp>> rec(2 * (size_t)n);
+  for (int q = 0; q < n; ++q) {
+    kg_handle* h = hs[q];
+    const State5 u0 = h->u, a0 = h->A;
+    const int px0 = h->px, p50 = h->p5, p250 = h->p25;
+    kg_status st = KG_OK;
+    for (int v = 0; v < 2 && st == KG_OK; ++v) {
+      h->rec = &rec[(size_t)v * n + q];
+      st = enqueue_step(h, scheme);
+      h->rec = nullptr;
+      advance_parity(h, scheme);
+    }
+    h->u = u0;
+    h->A = a0;
+    h->px = px0;
+    h->p5 = p50;
+    h->p25 = p250;
+    if (st != KG_OK) return st;
+  }
+  const size_t nops = rec[0].size();
+  for (size_t r = 0; r < rec.size(); ++r) {
+    if (rec[r].size() != nops) return fail(lead, KG_ERR_INVALID, "internal: batch step shapes differ");
+    for (size_t o = 0; o < nops; ++o)
+      if (rec[r][o].kind != rec[0][o].kind || rec[r][o].mode != rec[0][o].mode)
+        return fail(lead, KG_ERR_INVALID, "internal: batch step launches differ");
+  }
+  // descriptors: stage args and fill args per (version, launch, member), the
+  // workgroup prefix of each stage launch, the fill passes' block prefixes
+  std::vector<StageArgs> sd(2 * nops * n);
+  std::vector<FillArgs> fd(2 * nops * n);
+  std::vector<int> tstart(2 * nops * (n + 1), 0), total(2 * nops, 0), bstart(3 * (n + 1));
   std::vector<int64_t> nitems(3 * n);
   for (int v = 0; v < 2; ++v)
-    for (int q = 0; q < n; ++q) {
-      kg_handle* h = hs[q];
-      const State5& P = v ? h->A : h->u;
-      const State5& X = v ? h->u : h->A;
-      const Box fb = filled_box(h->G);
-      Roles rl[nst] = {roles(&P, nullptr, &X), roles(&P, nullptr, &h->Ua, nullptr, &X),
-                       roles(&P, nullptr, &h->Ub, nullptr, &h->Ua), roles(&P, &P, &X, &X, &h->Ub, &h->Ua)};
-      for (int st = 0; st < nst; ++st) {
-        StageArgs a = make_stage_args(h, modes[st], fb, rl[st]);
-        a.kc = h->tune_kc;  // each member keeps its own phi chunk
-        a.tj = lead->G.fd4 ? 8 : lead->tune_tj;  // the fd4 stages run 8 theta rows per workgroup
-        finalize_tiles(a);
-        sd[(v * nst + st) * n + q] = a;
+    for (size_t o = 0; o < nops; ++o) {
+      const size_t vo = (size_t)v * nops + o;
+      for (int q = 0; q < n; ++q) {
+        const RecOp& op = rec[(size_t)v * n + q][o];
+        if (op.kind == 0) {
+          StageArgs a = op.a;  // the member's own phi chunk; theta rows are the lead's
+          a.tj = lead->G.fd4 ? 8 : lead->tune_tj;
+          finalize_tiles(a);
+          sd[vo * n + q] = a;
+          tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + a.nti * a.ntj * a.ntk * a.nsys;
+        } else {
+          fd[vo * n + q] = op.f;
+        }
       }
-      FillArgs f{};
-      f.G = h->G;
-      for (int c = 0; c < 5; ++c) {
-        f.u[c] = X.f[c];
-        f.u_p[c] = P.f[c];
-      }
-      fd[v * n + q] = f;
+      total[vo] = tstart[vo * (n + 1) + n];
     }
-  tstart[0] = 0;
-  for (int q = 0; q < n; ++q) {
-    const StageArgs& a = sd[q];
-    tstart[q + 1] = tstart[q] + a.nti * a.ntj * a.ntk * a.nsys;
-  }
   for (int ph = 0; ph < 3; ++ph) {
     bstart[ph * (n + 1)] = 0;
     for (int q = 0; q < n; ++q) {
@@ -514,7 +532,9 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
   const int64_t* dni = (const int64_t*)(dev + o_ni);
   for (int it = 0; it < nsteps; ++it) {
     const int v = it & 1;
-    for (int st = 0; st < nst; ++st) {
+    for (size_t o = 0; o < nops; ++o) {
+      const size_t vo = (size_t)v * nops + o;
+      const int kind = rec[0][o].kind;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (lead->prof) {
         e0 = next_event(lead);
@@ -522,26 +542,18 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
         if (!e0 || !e1) return fail(lead, KG_ERR_HIP, "hipEventCreate failed");
         KG_HIP(lead, hipEventRecord(e0, lead->stream));
       }
-      KG_HIP(lead, launch_stage_batch(modes[st], dsd + (size_t)(v * nst + st) * n, dts, n, tstart[n],
-                                      lead->tune_tj, ef, lead->G.fd4, lead->stream));
+      if (kind == 0) {
+        KG_HIP(lead, launch_stage_batch(rec[0][o].mode, dsd + vo * n, dts + vo * (n + 1), n, total[vo],
+                                        lead->tune_tj, ef, lead->G.fd4, lead->stream));
+      } else {
+        for (int ph = 0; ph < 3; ++ph)
+          KG_HIP(lead, launch_fill_batch(dfd + vo * n, dbs + ph * (n + 1), n, dni + ph * n,
+                                         bstart[ph * (n + 1) + n], ph, lead->stream));
+      }
       if (lead->prof) {
         KG_HIP(lead, hipEventRecord(e1, lead->stream));
-        lead->ev_marks.push_back({0, lead->ev_used - 2});
+        lead->ev_marks.push_back({kind, lead->ev_used - 2});
       }
-    }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (lead->prof) {
-      e0 = next_event(lead);
-      e1 = next_event(lead);
-      if (!e0 || !e1) return fail(lead, KG_ERR_HIP, "hipEventCreate failed");
-      KG_HIP(lead, hipEventRecord(e0, lead->stream));
-    }
-    for (int ph = 0; ph < 3; ++ph)
-      KG_HIP(lead, launch_fill_batch(dfd + (size_t)v * n, dbs + ph * (n + 1), n, dni + ph * n,
-                                     bstart[ph * (n + 1) + n], ph, lead->stream));
-    if (lead->prof) {
-      KG_HIP(lead, hipEventRecord(e1, lead->stream));
-      lead->ev_marks.push_back({1, lead->ev_used - 2});
     }
   }
   // bookkeeping and ordering: later work on any member waits for the batch
@@ -550,10 +562,9 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
   for (int q = 0; q < n; ++q) {
     kg_handle* h = hs[q];
     if (h != lead) KG_HIP(lead, hipStreamWaitEvent(h->stream, lead->batch_done, 0));
-    for (int it = 0; it < nsteps; ++it) h->t = h->t + h->desc.dt;
-    if (nsteps & 1) {
-      std::swap(h->u, h->A);
-      h->px ^= 1;
+    for (int it = 0; it < nsteps; ++it) {
+      advance_parity(h, scheme);
+      h->t = h->t + h->desc.dt;
     }
   }
   return KG_OK;

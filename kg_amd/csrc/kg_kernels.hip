@@ -1255,6 +1255,13 @@ void finalize_tiles(StageArgs& a) {
 }
 
 template <int MODE>
+hipError_t launch_batch_mode(const StageArgs* descs, const int* tstart, int ngrids, int total, int tj, int ef,
+                             hipStream_t s) {
+  return tj == 8 ? launch_batch_tj<MODE, 8>(descs, tstart, ngrids, total, ef, s)
+                 : launch_batch_tj<MODE, 4>(descs, tstart, ngrids, total, ef, s);
+}
+
+template <int MODE>
 hipError_t launch_batch4(const StageArgs* descs, const int* tstart, int ngrids, int total, int ef, hipStream_t s) {
   dim3 grid((unsigned)total, 1, 1);
   dim3 block(TI, TJ4, 1);
@@ -1273,19 +1280,26 @@ hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstar
       case M_RK4_S2: return launch_batch4<M_RK4_S2>(descs, tstart, ngrids, total, ef, s);
       case M_RK4_S3: return launch_batch4<M_RK4_S3>(descs, tstart, ngrids, total, ef, s);
       case M_RK4_S4: return launch_batch4<M_RK4_S4>(descs, tstart, ngrids, total, ef, s);
-      default: return hipErrorInvalidValue;
+      case M_RK3_S1: return launch_batch4<M_RK3_S1>(descs, tstart, ngrids, total, ef, s);
+      case M_RK3_S2: return launch_batch4<M_RK3_S2>(descs, tstart, ngrids, total, ef, s);
+      case M_RK3_S3: return launch_batch4<M_RK3_S3>(descs, tstart, ngrids, total, ef, s);
+      default: return hipErrorInvalidValue;  // the PIRK stages are refused at the API
     }
   }
-  const bool t8 = tj == 8;
   switch (mode) {
-    case M_RK4_S1: return t8 ? launch_batch_tj<M_RK4_S1, 8>(descs, tstart, ngrids, total, ef, s)
-                             : launch_batch_tj<M_RK4_S1, 4>(descs, tstart, ngrids, total, ef, s);
-    case M_RK4_S2: return t8 ? launch_batch_tj<M_RK4_S2, 8>(descs, tstart, ngrids, total, ef, s)
-                             : launch_batch_tj<M_RK4_S2, 4>(descs, tstart, ngrids, total, ef, s);
-    case M_RK4_S3: return t8 ? launch_batch_tj<M_RK4_S3, 8>(descs, tstart, ngrids, total, ef, s)
-                             : launch_batch_tj<M_RK4_S3, 4>(descs, tstart, ngrids, total, ef, s);
-    case M_RK4_S4: return t8 ? launch_batch_tj<M_RK4_S4, 8>(descs, tstart, ngrids, total, ef, s)
-                             : launch_batch_tj<M_RK4_S4, 4>(descs, tstart, ngrids, total, ef, s);
+    case M_RK4_S1: return launch_batch_mode<M_RK4_S1>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_RK4_S2: return launch_batch_mode<M_RK4_S2>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_RK4_S3: return launch_batch_mode<M_RK4_S3>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_RK4_S4: return launch_batch_mode<M_RK4_S4>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_RK3_S1: return launch_batch_mode<M_RK3_S1>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_RK3_S2: return launch_batch_mode<M_RK3_S2>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_RK3_S3: return launch_batch_mode<M_RK3_S3>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_PIRK_1A: return launch_batch_mode<M_PIRK_1A>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_PIRK_1B: return launch_batch_mode<M_PIRK_1B>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_PIRK2_2: return launch_batch_mode<M_PIRK2_2>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_PIRK3_2A: return launch_batch_mode<M_PIRK3_2A>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_PIRK3_2B: return launch_batch_mode<M_PIRK3_2B>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_PIRK3_3: return launch_batch_mode<M_PIRK3_3>(descs, tstart, ngrids, total, tj, ef, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -79,3 +79,54 @@ def test_fd4_batch_equals_each_member_alone_and_the_oracle():
     mixed = [make_member(specs[0], operators="fd4"), make_member(specs[1])]
     with pytest.raises(KGError):
         batch_step(mixed, 1)
+
+
+@pytest.mark.parametrize("scheme,operators", [("rk3", "ref2"), ("pirk2", "ref2"), ("pirk3", "ref2"), ("rk3", "fd4"),
+                                              ("rk4", "ref2")])
+def test_every_scheme_batches_like_each_member_alone(scheme, operators):
+    """Batched launches for every integrator: members of uneven sizes (one a
+    complex state), a member stepped alone in between so the stale-Pi and
+    buffer parities differ across the batch, odd and even step counts; every
+    member ends bit-identical to its solo run."""
+    specs = _specs()
+    cx = [False, True, False, False]
+    batch = [make_member(s, scheme=scheme, operators=operators, is_complex=c) for s, c in zip(specs, cx)]
+    alone = [make_member(s, scheme=scheme, operators=operators, is_complex=c) for s, c in zip(specs, cx)]
+    print("batch", scheme, operators, flush=True)
+    batch_step(batch, 3)
+    batch[2].step(1)
+    batch_step(batch, 2)
+    for q, a in enumerate(alone):
+        a.step(6 if q == 2 else 5)
+    for q in range(len(specs)):
+        assert np.array_equal(batch[q].u_soa(), alone[q].u_soa()), (scheme, q)
+        assert batch[q].t == alone[q].t
+        if scheme in ("pirk2", "pirk3"):  # the stale-Pi stage arrays carry over too
+            assert np.array_equal(batch[q].u_stage_soa(0), alone[q].u_stage_soa(0)), (scheme, q)
+        batch[q].close()
+        alone[q].close()
+
+
+def test_pirk2_batch_matches_the_oracle():
+    """The reference's live integrator in batched launches against the oracle."""
+    specs = _specs()
+    batch = [make_member(s, scheme="pirk2") for s in specs]
+    u0 = [b.u_soa()[0] for b in batch]
+    print("pirk2 batch vs oracle", flush=True)
+    batch_step(batch, 4)
+    for q, s in enumerate(specs):
+        sim = oracle.Sim(s.Nr, s.Ntheta + 1, s.Nphi + 1, 2, s.xmin, s.xmax, "Kerr_Schild", 0.0, s.M_bh, s.m,
+                         "outflow", 0.25)
+        sim.u[...] = u0[q]
+        sim.step("pirk2", 4)
+        assert np.array_equal(batch[q].u_soa()[0], sim.u), q
+
+
+def test_batch_refuses_mixed_schemes():
+    from kg_amd import KGError
+    from kg_amd import _capi as capi
+    specs = _specs()
+    mixed = [make_member(specs[0], scheme="rk4"), make_member(specs[1], scheme="pirk2")]
+    with pytest.raises(KGError) as ei:
+        batch_step(mixed, 1)
+    assert ei.value.status == capi.KG_ERR_UNSUPPORTED
