@@ -6,13 +6,16 @@
 !   in.bin/out.bin: u(5, -g:Nx+g, -g:Ny+g, 0:Nz) complex(8), raw stream
 !   in.bin may instead be ic=<type>,<mu>,<sigma>,<ncart>,<seed>: the initial
 !   data of main/initial_conditions.f90 built by the engine
+!   optional 11th argument batch=<K>: K copies of the grid advanced together in
+!   batched launches; the driver stops with an error unless all K agree
 program kg_evolve
   use kg_hip_mod
   implicit none
   type(kg_lines_method) :: MoL
-  complex(kind=8), allocatable :: u(:,:,:,:)
+  type(kg_lines_method), allocatable :: batch(:)
+  complex(kind=8), allocatable :: u(:,:,:,:), v(:,:,:,:)
   character(len=256) :: arg, fin, fout, scheme
-  integer :: Nx, Ny, Nz, nsteps, l, g, isch, ncart, ops
+  integer :: Nx, Ny, Nz, nsteps, l, g, isch, ncart, ops, nb, q
   integer(kind=8) :: seed
   real(kind=8) :: xmin, xmax, m, mu, sigma, amp
   character(len=64) :: ictype
@@ -28,6 +31,11 @@ program kg_evolve
   call get_command_argument(8, arg); read(arg, *) nsteps
   call get_command_argument(9, fin)
   call get_command_argument(10, fout)
+  nb = 1
+  if (command_argument_count() >= 11) then
+    call get_command_argument(11, arg)
+    if (arg(1:6) == 'batch=') read(arg(7:), *) nb
+  end if
 
   ops = KG_OPS_REF2
   select case (trim(scheme))
@@ -53,16 +61,43 @@ program kg_evolve
     call MoL%upload(u)
   end if
 
-  do l = 1, nsteps
-    select case (isch)
-    case (KG_SCHEME_PIRK2); call MoL%pirk2()
-    case (KG_SCHEME_PIRK3); call MoL%pirk3()
-    case (KG_SCHEME_RK3);   call MoL%rk3()
-    case default;    call MoL%rk4()
-    end select
-  end do
+  if (nb > 1) then
+    allocate(batch(nb))
+    batch(1) = MoL
+    do q = 2, nb
+      batch(q) = kg_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
+                                             'outflow', 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., ops)
+      if (fin(1:3) == 'ic=') then
+        call batch(q)%initial_conditions(trim(ictype), mu, sigma, ncart, seed, amp)
+      else
+        call batch(q)%upload(u)
+      end if
+    end do
+    call kg_lines_method_batch_step(batch, nsteps)
+    MoL = batch(1)
+    call MoL%download(u)
+    allocate(v, mold=u)
+    do q = 2, nb
+      call batch(q)%download(v)
+      if (any(v /= u)) then
+        print '(A,I4,A)', 'batch member ', q, ' differs from member 1'
+        stop 1
+      end if
+      call batch(q)%destroy()
+    end do
+    print '(A,I4,A)', 'batch of ', nb, ' members agree'
+  else
+    do l = 1, nsteps
+      select case (isch)
+      case (KG_SCHEME_PIRK2); call MoL%pirk2()
+      case (KG_SCHEME_PIRK3); call MoL%pirk3()
+      case (KG_SCHEME_RK3);   call MoL%rk3()
+      case default;    call MoL%rk4()
+      end select
+    end do
+    call MoL%download(u)
+  end if
 
-  call MoL%download(u)
   open(unit=11, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(11) u
   close(11)

@@ -128,6 +128,12 @@ module kg_hip_mod
       type(c_ptr), value :: h
       integer(c_int32_t) :: st
     end function kg_sync
+    function kg_batch_step(hs, n, nsteps) bind(C, name="kg_batch_step") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), intent(in) :: hs(*)
+      integer(c_int32_t), value :: n, nsteps
+      integer(c_int32_t) :: st
+    end function kg_batch_step
     function kg_initial_data(h, desc, amp) bind(C, name="kg_initial_data") result(st)
       import :: kg_ic_desc, c_ptr, c_int32_t, c_double
       type(c_ptr), value :: h
@@ -144,7 +150,7 @@ module kg_hip_mod
   public :: kg_initial_data, kg_ic_last_error
   public :: kg_create, kg_destroy, kg_last_error, kg_get_time, kg_set_time, kg_upload_state, &
             kg_download_state, kg_upload_aux, kg_download_aux, kg_fill_ghosts, kg_pirk2, kg_pirk3, &
-            kg_rk3, kg_rk4, kg_step, kg_sync
+            kg_rk3, kg_rk4, kg_step, kg_sync, kg_batch_step, kg_lines_method_batch_step
 
   ! The reference's lines_method as seen by its driver.
   type, public :: kg_lines_method
@@ -261,6 +267,24 @@ contains
     call kg_check(kg_get_time(this%h, t, dt), this%h)
     this%t = t
   end subroutine sync_time
+
+  ! Several grids advanced together (no counterpart in the reference, which
+  ! runs one grid per process): nsteps steps of their common scheme, one
+  ! kernel launch per stage over all members; each ends as if stepped alone.
+  subroutine kg_lines_method_batch_step(members, nsteps)
+    type(kg_lines_method), intent(inout) :: members(:)
+    integer, intent(in) :: nsteps
+    type(c_ptr), allocatable :: hs(:)
+    integer :: q
+    allocate(hs(size(members)))
+    do q = 1, size(members)
+      hs(q) = members(q)%h
+    end do
+    call kg_check(kg_batch_step(hs, int(size(members), c_int32_t), int(nsteps, c_int32_t)), members(1)%h)
+    do q = 1, size(members)
+      call sync_time(members(q))
+    end do
+  end subroutine kg_lines_method_batch_step
 
   subroutine klm_pirk2(this)
     class(kg_lines_method), intent(inout) :: this

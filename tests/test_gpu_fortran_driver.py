@@ -19,8 +19,10 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built (no amdflang)")]
 
 
-@pytest.mark.parametrize("scheme", ["pirk2", "rk4", "rk4_fd4"])
-def test_fortran_driver_matches_oracle(tmp_path, scheme):
+@pytest.mark.parametrize("scheme,batch", [("pirk2", 1), ("rk4", 1), ("rk4_fd4", 1), ("pirk2", 3), ("rk4_fd4", 2)])
+def test_fortran_driver_matches_oracle(tmp_path, scheme, batch):
+    """batch > 1: the driver advances that many copies through
+    kg_lines_method_batch_step (kg_batch_step) and checks they agree."""
     Nx, Ny, Nz, xmin, xmax, m = 20, 9, 17, 1.8, 10.0, 0.2
     re = shell_state(Nx, Ny, Nz, 2, xmin, xmax, 5.0, 1.2, 1.0, {(2, 0): 1.0, (1, -1): 0.4})
     im = shell_state(Nx, Ny, Nz, 2, xmin, xmax, 6.0, 1.0, 0.3, {(1, 1): 1.0})
@@ -29,10 +31,13 @@ def test_fortran_driver_matches_oracle(tmp_path, scheme):
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     uf.ravel(order="F").tofile(fin)
     nsteps = 6
-    print("fortran driver", scheme, flush=True)
+    print("fortran driver", scheme, "batch", batch, flush=True)
+    extra = ["batch=%d" % batch] if batch > 1 else []
     r = subprocess.run([EXE, str(Nx), str(Ny), str(Nz), repr(xmin), repr(xmax), repr(m), scheme, str(nsteps),
-                        str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+                        str(fin), str(fout)] + extra, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    if batch > 1:
+        assert "batch of %4d members agree" % batch in r.stdout, r.stdout
     got = np.fromfile(fout, dtype=np.complex128).reshape(uf.shape, order="F")
     got = np.transpose(got, (0, 3, 2, 1))
     for part, arr in (("re", re), ("im", im)):
