@@ -172,7 +172,8 @@ class GpuEngine:
     projector; ``step`` is one batched launch per RK4 stage."""
 
     def __init__(self, lmax: int = 8, nk: int = 512, kmax: float = 4.0, CFL: float = 0.25, tuning=None,
-                 on_device: bool = True, operators: str = "ref2", batch_kc=None, fill_workgroups: int = 4096):
+                 on_device: bool = True, operators: str = "ref2", batch_kc=None, fill_workgroups: int = 4096,
+                 scheme: str = "rk4"):
         """tuning: (kc, tj) of a request run alone; batch_kc: the phi chunk of
         requests advanced together (defaults to tuning's), or "fill" to pick it
         per chunk from the active set (``autotune.fill_kc`` with
@@ -181,12 +182,14 @@ class GpuEngine:
         self.tuning = tuning
         self.batch_kc = batch_kc
         self.fill_workgroups = int(fill_workgroups)
+        self.scheme = scheme  # every request runs this integrator
         self.kc_chunks = {}  # phi chunk -> batched calls that used it
         self.on_device = on_device
         self.operators = operators
 
     def bytes_for(self, spec: MemberSpec) -> int:
-        """Device footprint: four RK4 state buffers (u, next state / k1, k2, k3),
+        """Device footprint: four state buffers (RK4's u, next state / k1, k2, k3;
+        the other schemes hold fewer, so this bounds them too),
         the metric tables and the spectral tables and scratch."""
         ni, nj, nk = spec.Nr + 5, spec.Ntheta + 1 + 5, spec.Nphi + 2
         ld = (ni + 15) // 16 * 16
@@ -202,7 +205,8 @@ class GpuEngine:
         from .batch import make_member
         from .spectral import SpectralProjector
 
-        lm = make_member(req.spec, CFL=self.CFL, on_device=self.on_device, operators=self.operators)
+        lm = make_member(req.spec, CFL=self.CFL, on_device=self.on_device, operators=self.operators,
+                         scheme=self.scheme)
         if self.tuning:
             lm.set_tuning(*self.tuning)
         sp = SpectralProjector(lm, lmax=self.lmax, nk=self.nk, basis="uniform", kmax=self.kmax)

@@ -60,3 +60,19 @@ def test_fd4_served_with_per_chunk_phi_chunks_equal_running_alone():
         fields, spectrum = eng.run_alone(req)
         assert np.array_equal(res.fields, fields), req.rid
         assert np.array_equal(res.spectrum, spectrum), req.rid
+
+
+def test_pirk2_served_results_equal_running_alone():
+    """The reference's live integrator through the serving loop."""
+    eng = GpuEngine(lmax=4, nk=16, kmax=4.0, on_device=False, scheme="pirk2")
+    reqs = _requests()
+    sizes = sorted(eng.bytes_for(r.spec) for r in reqs)
+    loop = ServingLoop(eng, sizes[-1] + sizes[-2], chunk=4)
+    print("serving %d pirk2 requests" % len(reqs), flush=True)
+    results, stats = loop.run(reqs)
+    assert stats.joined_running >= 1 and max(stats.batch_sizes) >= 2
+    for req, res in zip(reqs, results):
+        assert not res.error
+        fields, spectrum = eng.run_alone(req)
+        assert np.array_equal(res.fields, fields), req.rid
+        assert np.array_equal(res.spectrum, spectrum), req.rid
