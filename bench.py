@@ -19,10 +19,11 @@ launch per RK4 stage, vs the same grids one at a time) with its
 spherical-Bessel projection (l <= 8, 512 k, split BF16 and FP64), the serving
 loop (Poisson requests, continuous batching, p50/p99 time-to-result), and
 the reference's random_shell initial data built on the GPU for the full grid,
-and the CPU oracle on a bounded sample.
+the reference's own production step (PIRK2 with its 2nd-order stencils) on the
+full grid, and the CPU oracle on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--operators fd4|ref2] [--other-steps K]
-                [--batched N] [--serve N] [--ic N] [--ic-only]
+                [--pirk2-steps K] [--batched N] [--serve N] [--ic N] [--ic-only]
 """
 from __future__ import annotations
 
@@ -37,9 +38,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 B_PER_UPDATE = {"rk4": 680, "rk3": 320, "pirk2": 248, "pirk3": 416}
-# phi chunks the plain path's autotuner chooses from for a grid run by itself
-# (a grid of 256 x 16 x 32 is fastest at 2, one of 1024 x 16 x 32 and larger at 8)
-PLAIN_KC = (1, 2, 4, 8, 16)
+# launch shapes (phi chunk, theta rows) the plain path's autotuner chooses from
+# for a grid run by itself (with the fd4 set a grid of 256 x 16 x 32 is fastest
+# at 2 planes, one of 1024 x 16 x 32 and larger at 8; the fd4 stages always run
+# 8 theta rows)
+PLAIN_SHAPES = [(kc, tj) for tj in (4, 8) for kc in (1, 2, 4, 8, 16)]
 STAGES = {"rk4": 4, "rk3": 3, "pirk2": 3, "pirk3": 5}
 
 CFG = dict(Nx=2048, Ny=129, Nz=257, xmin=1.8, xmax=200.0, m=0.1, M_bh=1.0, a=0.0, CFL=0.25, scheme="rk4")
@@ -115,7 +118,7 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     in {(16,32), (32,64), (64,128)}, m ~ U[0,0.5], M ~ U[0.5,2], RK4), advanced
     `steps` steps (a) all together, one launch per stage (kg_batch_step), and
     (b) one at a time through the plain path (kg_step graph replay per grid,
-    each grid at the phi chunk its own autotuner picks among PLAIN_KC).
+    each grid at the launch shape its own autotuner picks among PLAIN_SHAPES).
     Aggregate grid-point updates/s of each; inputs resident in HBM."""
     import torch
     from kg_amd.autotune import autotune, autotune_batch
@@ -137,8 +140,8 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     # the timed region), then a warm-up step (graph capture)
     picks = {}
     for m in members:
-        best, _ = autotune(m, candidates=[(kc, tuning[1]) for kc in PLAIN_KC], steps=2)
-        picks[best[0]] = picks.get(best[0], 0) + 1
+        best, _ = autotune(m, candidates=PLAIN_SHAPES, steps=2)
+        picks["%d/%d" % best] = picks.get("%d/%d" % best, 0) + 1
         m.step(1)
         m.sync()
     tp = 0.0
@@ -152,7 +155,7 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
         m.close()
     vb, vp = pts * steps / tb, pts * steps / tp
     return {"members": n, "steps": steps, "operators": operators, "batch_kc": batch_kc,
-            "plain_tuning": "autotuned per grid", "plain_kc_picks": {str(k): v for k, v in sorted(picks.items())},
+            "plain_tuning": "autotuned per grid", "plain_picks_kc_tj": dict(sorted(picks.items())),
             "grid_points": pts, "batched_updates_per_s": vb,
             "plain_updates_per_s": vp, "speedup": vb / vp,
             "workload": "uneven RK4 batch: Nr~U{256..2048}, (Ntheta,Nphi) in {(16,32),(32,64),(64,128)}, "
@@ -234,7 +237,7 @@ def serving(nreq, tuning, operators, batch_kc=None):
     alone = {}
     for r in reqs:
         member = eng.admit(r)
-        alone[r.rid], _ = autotune(member[0], candidates=[(kc, tuning[1]) for kc in PLAIN_KC], steps=2)
+        alone[r.rid], _ = autotune(member[0], candidates=PLAIN_SHAPES, steps=2)
         eng.release(member)
     t0 = time.perf_counter()
     for r in reqs:
@@ -257,10 +260,12 @@ KERNELS = {"ref2": "stage_kernel (RHS + RK4 stage combine, the reference's 2nd-o
            "fd4": "stage4_kernel (RHS + RK4 stage combine, the 4th-order stencil set)"}
 
 
-def plain(operators, steps, warmup, kc=0, tj=8):
-    """The plain single-grid workload with one operator set: state resident in
-    HBM, launch shape autotuned (or fixed), `warmup` untimed steps, `steps`
-    timed steps with HIP events around every stage kernel."""
+def plain(operators, steps, warmup, kc=0, tj=8, scheme=None):
+    """The plain single-grid workload with one operator set (and the workload's
+    RK4 unless `scheme` names another integrator): state resident in HBM,
+    launch shape autotuned (or fixed), `warmup` untimed steps, `steps` timed
+    steps with HIP events around every stage kernel."""
+    scheme = scheme or CFG["scheme"]
     import numpy as np
     import torch
     import kg_amd
@@ -268,7 +273,7 @@ def plain(operators, steps, warmup, kc=0, tj=8):
 
     print("bench: building initial data (%s)" % operators, file=sys.stderr, flush=True)
     lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
-                            CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=CFG["scheme"],
+                            CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=scheme,
                             operators=operators)
     u0 = shell_state_device(CFG["Nx"], CFG["Ny"], CFG["Nz"], 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0,
                             random_lm_coeffs(20261203))
@@ -314,15 +319,18 @@ def plain(operators, steps, warmup, kc=0, tj=8):
     lm.close()
     npts = CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1)
     elapsed = t1 - t0
-    sch = CFG["scheme"]
+    sch = scheme
     per_launch_bytes = npts * B_PER_UPDATE[sch] / STAGES[sch]
     avg_stage_s = stage_ms / 1e3 / max(stage_n, 1)
     achieved = per_launch_bytes / avg_stage_s / 1e9
-    return {"operators": operators, "value": npts * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+    return {"operators": operators, "scheme": sch, "value": npts * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
             "steps": steps, "state_finite": finite, "tuning": {"kc": best[0], "tj": best[1]},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(best, operators),
-                         "kernel": KERNELS[operators], "avg_kernel_ms": avg_stage_s * 1e3, "launches": stage_n,
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(best, operators) if sch == CFG["scheme"] else None,
+                         "kernel": KERNELS[operators] if sch == CFG["scheme"] else
+                         "stage_kernel (%s stages, the reference's 2nd-order stencils)" % sch.upper(),
+                         "avg_kernel_ms": avg_stage_s * 1e3, "launches": stage_n,
                          "bytes_per_launch": per_launch_bytes},
             "fill_ms_per_step": fill_ms / steps,
             "host_transfer": {"bytes": host_bytes, "download_s": t3 - t2, "upload_s": t4 - t3,
@@ -342,6 +350,8 @@ def main():
                          "ref2 (the reference's live 2nd-order stencils); the other is timed beside it")
     ap.add_argument("--other-steps", type=int, default=100,
                     help="timed steps of the other operator set (0: skip)")
+    ap.add_argument("--pirk2-steps", type=int, default=50,
+                    help="timed steps of the reference's PIRK2 with its 2nd-order stencils on the full grid (0: skip)")
     ap.add_argument("--kc", type=int, default=0, help="fix the stage kernel's phi planes per workgroup (skips autotune)")
     ap.add_argument("--tj", type=int, default=8, help="theta rows per workgroup when --kc is given")
     ap.add_argument("--batched", type=int, default=256,
@@ -392,11 +402,14 @@ def main():
     if args.other_steps > 0:
         print("bench: the %s operator set beside it" % other, file=sys.stderr, flush=True)
         out["plain_" + other] = plain(other, args.other_steps, 5)
+    if args.pirk2_steps > 0:
+        print("bench: the reference's PIRK2 + 2nd-order step", file=sys.stderr, flush=True)
+        out["plain_ref2_pirk2"] = plain("ref2", args.pirk2_steps, 5, scheme="pirk2")
     if args.ic > 0:
         print("bench: initial data, ncart %d" % args.ic, file=sys.stderr, flush=True)
         out["initial_data"] = initial_data(args.ic)
     # the batched and serving legs run the timed workload's operator set; a
-    # grid run alone uses the phi chunk its own autotuner picks (PLAIN_KC),
+    # grid run alone uses the launch shape its own autotuner picks (PLAIN_SHAPES),
     # batched members the chunk the batch autotuner picks, served ones the
     # chunk the workgroup-fill rule gives for the requests active in each call;
     # theta rows per workgroup are 8 throughout
