@@ -48,7 +48,7 @@ STAGES = {"rk4": 4, "rk3": 3, "pirk2": 3, "pirk3": 5}
 CFG = dict(Nx=2048, Ny=129, Nz=257, xmin=1.8, xmax=200.0, m=0.1, M_bh=1.0, a=0.0, CFL=0.25, scheme="rk4")
 
 
-def cpu_baseline(operators, max_seconds=20.0):
+def cpu_baseline(operators, max_seconds=12.0):
     """The oracle (C restatement, OpenMP over the host threads) on a bounded
     sample of the workload: the same radial/polar grid (Nr = 2048, Ntheta =
     128) with 16 azimuthal planes, RK4 steps with the same operator set until
@@ -67,7 +67,7 @@ def cpu_baseline(operators, max_seconds=20.0):
         sim.step("rk4", 1)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= max_seconds or steps >= 64:
+        if el >= max_seconds or steps >= 1024:
             break
     ups = Nx * (Ny - 1) * (Nz - 1) * steps / el
     nt = oracle.threads()
