@@ -894,7 +894,8 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
     bool ndir = false;
     Src qn{0u, 0};
     constexpr int AHEAD = PF3 ? 3 : 2;
-    if (valid && kk + AHEAD <= Nz) {
+    // (with PF3, plane kk+3 serves the next iteration only)
+    if (valid && kk + AHEAD <= Nz && (!PF3 || kk + 1 < k1)) {
       qn = remap(ii, jj, kk + AHEAD);
       ndir = qn.kind != 0;
       fetch(4, qn, nr[0]);
