@@ -158,9 +158,11 @@ kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
 kg_status kg_sync(kg_handle* h);
 
 /* Launch shape of the stage kernel (no reference counterpart): kc = phi
- * planes each workgroup marches over, tj = theta rows per workgroup (4 or 8;
- * the fd4 stages always run 8).  Results do not depend on it; autotuning
- * callers sweep it.  Setting the current shape again is a no-op. */
+ * planes each workgroup marches over at most (the ceil(Nz/kc) chunks of a
+ * tile column are balanced: 257 planes at kc = 32 march as 9 chunks of 29),
+ * tj = theta rows per workgroup (4 or 8; the fd4 stages always run 8).
+ * Results do not depend on it; autotuning callers sweep it.  Setting the
+ * current shape again is a no-op. */
 kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj);
 
 /* Kernel timing with HIP events on the handle's stream (no reference
