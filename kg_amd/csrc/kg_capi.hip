@@ -485,7 +485,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
         const RecOp& op = rec[(size_t)v * n + q][o];
         if (op.kind == 0) {
           StageArgs a = op.a;  // the member's own phi chunk; theta rows are the lead's
-          a.tj = lead->G.fd4 ? 8 : lead->tune_tj;
+          a.tj = lead->G.fd4 ? 8 : stage_tj(op.mode, lead->tune_tj);
           finalize_tiles(a);
           sd[vo * n + q] = a;
           tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + a.nti * a.ntj * a.ntk * a.nsys;
@@ -544,7 +544,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
       }
       if (kind == 0) {
         KG_HIP(lead, launch_stage_batch(rec[0][o].mode, dsd + vo * n, dts + vo * (n + 1), n, total[vo],
-                                        lead->tune_tj, ef, lead->G.fd4, lead->stream));
+                                        stage_tj(rec[0][o].mode, lead->tune_tj), ef, lead->G.fd4, lead->stream));
       } else {
         for (int ph = 0; ph < 3; ++ph)
           KG_HIP(lead, launch_fill_batch(dfd + vo * n, dbs + ph * (n + 1), n, dni + ph * n,

@@ -123,8 +123,11 @@ hipError_t launch_stage(int mode, const StageArgs& a, hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase /*0 x,1 y,2 z*/, int nsys);
 // Batched launches: one launch covers every grid of descs[0..ngrids).
 // Stage: grid q owns workgroups [tstart[q], tstart[q+1]), its tiles times its
-// systems (finalize_tiles sets the tile counts).  RK4 stages only.
+// systems (finalize_tiles sets the tile counts); any stage of any scheme.
 void finalize_tiles(StageArgs& a);
+// theta rows per workgroup a 2nd-order stage runs with for the tuned `tj`
+// (RK4 stage 4 always 4)
+int stage_tj(int mode, int tj);
 hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int tj,
                               int ef, int fd4, hipStream_t s);
 // Fill pass `phase`: grid q owns blocks [bstart[q], ...) of 256 work items, nitems[q] items.

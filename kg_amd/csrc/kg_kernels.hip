@@ -1091,10 +1091,12 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
 // One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The
 // second launch bound is the minimum number of waves per SIMD.  RK4 stages 2
 // and 3 ask for 4 (128 VGPRs; 2 over their natural allocation).  The final
-// stage asks for 2 at TJ = 8 (no constraint: 3 waves with spills measured
-// slower than 2 without) and 4 at TJ = 4.
+// stage asks for 2 (its natural ~184 VGPRs; 3 or 4 waves spill) and always
+// runs 4 theta rows per workgroup, so two workgroups with their own barriers
+// share a CU (3.25 -> 3.14 ms against one 8-row workgroup on the
+// 2048 x 128 x 256 grid).
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel(const StageArgs a) {
   stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
 }
 
@@ -1113,7 +1115,7 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ start, int n, i
 }
 
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 16 / TJ : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel_batch(
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel_batch(
     const StageArgs* __restrict__ descs, const int* __restrict__ tstart, int ngrids, int total) {
   const int b = remap_xcd((int)blockIdx.x, total);
   const int q = find_grid(tstart, ngrids, b);
@@ -1278,6 +1280,8 @@ hipError_t launch_batch_tj(const StageArgs* descs, const int* tstart, int ngrids
 
 }  // namespace
 
+int stage_tj(int mode, int tj) { return mode == M_RK4_S4 ? 4 : tj; }
+
 void finalize_tiles(StageArgs& a) {
   if (a.tj != 4 && a.tj != 8) a.tj = 4;
   if (a.kc < 1) a.kc = 1;
@@ -1355,6 +1359,7 @@ hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
       default: return hipErrorInvalidValue;  // the PIRK stages are refused at the API
     }
   }
+  a.tj = stage_tj(mode, a.tj);
   finalize_tiles(a);
   switch (mode) {
     case M_RK4_S1: return launch_mode<M_RK4_S1>(a, s, nsys);
