@@ -23,7 +23,7 @@ the reference's own production step (PIRK2 with its 2nd-order stencils) on the
 full grid, and the CPU oracle on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--operators fd4|ref2] [--other-steps K]
-                [--pirk2-steps K] [--batched N] [--serve N] [--ic N] [--ic-only]
+                [--pirk2-steps K] [--batched N] [--serve N] [--serve-small N] [--ic N] [--ic-only]
 """
 from __future__ import annotations
 
@@ -216,16 +216,17 @@ def initial_data(ncart, reps=3):
             "amp": amp, "state": "complex, 5 fields on the full grid"}
 
 
-def serving(nreq, tuning, operators, batch_kc=None):
-    """Serving loop: nreq requests of the batched workload arriving as a Poisson
-    stream (50/s), steps ~U{500..5000}, admitted against the free HBM and run by
-    continuous batching; each returns its final fields and its spectrum.  The
-    same requests one at a time through the plain path give the comparison."""
+def serving(nreq, tuning, operators, batch_kc=None, specs=None, workload=None):
+    """Serving loop: nreq requests of the batched workload (or of `specs`)
+    arriving as a Poisson stream (50/s), steps ~U{500..5000}, admitted against
+    the free HBM and run by continuous batching; each returns its final fields
+    and its spectrum.  The same requests one at a time through the plain path
+    give the comparison."""
     import torch
     from kg_amd.autotune import autotune
     from kg_amd.serving import GpuEngine, ServingLoop, percentile, poisson_requests
 
-    reqs = poisson_requests(nreq)
+    reqs = poisson_requests(nreq, specs=specs)
     eng = GpuEngine(tuning=tuning, operators=operators, batch_kc=batch_kc)
     free, _ = torch.cuda.mem_get_info()
     budget = int(0.85 * free)
@@ -252,8 +253,8 @@ def serving(nreq, tuning, operators, batch_kc=None):
             "served_updates_per_s": stats.updates_per_s, "one_at_a_time_updates_per_s": ups_plain,
             "speedup": stats.updates_per_s / ups_plain, "max_active": stats.max_active,
             "chunks": stats.chunks, "joined_running": stats.joined_running,
-            "workload": "Poisson(50/s) requests drawn as the batched workload, steps ~U{500..5000}, "
-                        "chunks of 50 RK4 steps, final fields + l<=8, 512-k spectrum (split-BF16)"}
+            "workload": workload or ("Poisson(50/s) requests drawn as the batched workload, steps ~U{500..5000}, "
+                                     "chunks of 50 RK4 steps, final fields + l<=8, 512-k spectrum (split-BF16)")}
 
 
 KERNELS = {"ref2": "stage_kernel (RHS + RK4 stage combine, the reference's 2nd-order stencils)",
@@ -360,6 +361,8 @@ def main():
     ap.add_argument("--ic", type=int, default=512,
                     help="Cartesian points per axis of the random_shell initial data to time (0: skip)")
     ap.add_argument("--ic-only", action="store_true", help="time only the initial data and exit")
+    ap.add_argument("--serve-small", type=int, default=64,
+                    help="requests of the population's smallest class through the serving loop (0: skip)")
     ap.add_argument("--serve", type=int, default=8,
                     help="requests of the serving loop to run beside the plain path (0: skip)")
     args = ap.parse_args()
@@ -427,6 +430,13 @@ def main():
     if args.serve > 0:
         print("bench: serving loop, %d requests" % args.serve, file=sys.stderr, flush=True)
         out["serving"] = serving(args.serve, shape, args.operators, "fill")
+    if args.serve_small > 0:
+        print("bench: serving loop, %d small requests" % args.serve_small, file=sys.stderr, flush=True)
+        out["serving_small_requests"] = serving(
+            args.serve_small, shape, args.operators, "fill", specs=small_specs(args.serve_small, seed=20261205),
+            workload="Poisson(50/s) requests of the population's smallest class (Nr=256, (Ntheta,Nphi)=(16,32), "
+                     "m~U[0,0.5], M~U[0.5,2]), steps ~U{500..5000}, chunks of 50 RK4 steps, final fields + "
+                     "l<=8, 512-k spectrum (split-BF16)")
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.operators)

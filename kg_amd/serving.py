@@ -75,11 +75,12 @@ class ServeStats:
         return self.updates / self.wall if self.wall > 0 else 0.0
 
 
-def poisson_requests(n: int, rate: float = 50.0, seed: int = 20261203, steps=(500, 5000)):
-    """n requests of the uneven batched workload arriving as a Poisson stream of
-    `rate` per second, step counts uniform in [steps[0], steps[1]]."""
+def poisson_requests(n: int, rate: float = 50.0, seed: int = 20261203, steps=(500, 5000), specs=None):
+    """n requests of the uneven batched workload (or of the grids `specs`)
+    arriving as a Poisson stream of `rate` per second, step counts uniform in
+    [steps[0], steps[1]]."""
     rng = np.random.default_rng(seed + 1)
-    specs = uneven_specs(n, seed)
+    specs = uneven_specs(n, seed) if specs is None else list(specs)[:n]
     gaps = rng.exponential(1.0 / rate, size=n)
     arrivals = np.cumsum(gaps) - gaps[0]
     nsteps = rng.integers(steps[0], steps[1] + 1, size=n)
