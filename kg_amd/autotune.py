@@ -49,7 +49,7 @@ def autotune(lm, candidates=CANDIDATES, steps=2, verbose=False):
     return best, timings
 
 
-def autotune_batch(members, candidates=(8, 16, 32), steps=2, tj=8):
+def autotune_batch(members, candidates=(8, 16, 32, 64), steps=2, tj=8):
     """Pick one phi-chunk length for a batch (every member gets it) by timing
     `steps` batched steps per candidate after one warm-up step.  The members
     advance (1 + steps) steps per candidate: tune before the work whose state
@@ -87,9 +87,9 @@ def fill_kc(members, candidates=(8, 16, 32), tj=8, min_workgroups=4096):
     chunks of work per stage over the batch (16 per CU), else the shortest
     candidate.  Longer chunks re-read fewer phi edges; shorter ones keep the
     CUs busy when few or small grids are active.  On one MI355X the rule gives
-    the timed picks of `autotune` / `autotune_batch`: 32 for one 2048 x 128 x 256
-    grid and for the 256-member uneven population, 16 for 256 grids of
-    256 x 16 x 32."""
+    the timed picks of `autotune` / `autotune_batch` among 8, 16 and 32 planes:
+    32 for one 2048 x 128 x 256 grid and for the 256-member uneven population,
+    16 for 256 grids of 256 x 16 x 32."""
     work = sum(stage_tile_planes(m, tj) for m in members)
     cands = sorted(candidates)
     for kc in reversed(cands):
