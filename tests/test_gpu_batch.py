@@ -130,3 +130,24 @@ def test_batch_refuses_mixed_schemes():
     with pytest.raises(KGError) as ei:
         batch_step(mixed, 1)
     assert ei.value.status == capi.KG_ERR_UNSUPPORTED
+
+
+def test_batch_autotuner_picks_a_candidate_and_results_stay_solo_identical():
+    """autotune_batch times its candidates on the batch itself (advancing it
+    1 + steps steps per candidate) and leaves every member at the pick; the
+    members still match their solo runs step for step afterwards."""
+    from kg_amd.autotune import autotune_batch
+    specs = _specs()
+    batch = [make_member(s, operators="fd4") for s in specs]
+    alone = [make_member(s, operators="fd4") for s in specs]
+    print("batch autotune", flush=True)
+    kc, timings = autotune_batch(batch, candidates=(2, 8, 64), steps=1)
+    assert kc in (2, 8, 64) and set(timings) == {2, 8, 64}
+    assert all(t > 0 for t in timings.values())
+    batch_step(batch, 2)
+    for a in alone:
+        a.step(3 * 2 + 2)  # three candidates x (1 warm-up + 1 timed) + 2
+    for q in range(len(specs)):
+        assert np.array_equal(batch[q].u_soa(), alone[q].u_soa()), q
+        batch[q].close()
+        alone[q].close()
