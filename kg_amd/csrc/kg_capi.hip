@@ -495,11 +495,13 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
       }
       total[vo] = tstart[vo * (n + 1) + n];
     }
+  int wide[3] = {0, 0, 0};  // a pass with 2^32 items or more in some grid
   for (int ph = 0; ph < 3; ++ph) {
     bstart[ph * (n + 1)] = 0;
     for (int q = 0; q < n; ++q) {
       const int64_t it = fill_work_items(hs[q]->G, hs[q]->nsys, ph);
       nitems[ph * n + q] = it;
+      if (it >= (int64_t{1} << 32)) wide[ph] = 1;
       bstart[ph * (n + 1) + q + 1] = bstart[ph * (n + 1) + q] + (int)((it + 255) / 256);
     }
   }
@@ -548,7 +550,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
       } else {
         for (int ph = 0; ph < 3; ++ph)
           KG_HIP(lead, launch_fill_batch(dfd + vo * n, dbs + ph * (n + 1), n, dni + ph * n,
-                                         bstart[ph * (n + 1) + n], ph, lead->stream));
+                                         bstart[ph * (n + 1) + n], ph, wide[ph], lead->stream));
       }
       if (lead->prof) {
         KG_HIP(lead, hipEventRecord(e1, lead->stream));

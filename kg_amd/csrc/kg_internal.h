@@ -132,8 +132,9 @@ hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstar
                               int ef, int fd4, hipStream_t s);
 // Fill pass `phase`: grid q owns blocks [bstart[q], ...) of 256 work items, nitems[q] items.
 int64_t fill_work_items(const Geom& G, int nsys, int phase);
+// wide = 1 when some grid's pass has 2^32 items or more (64-bit index arithmetic)
 hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrids, const int64_t* nitems,
-                             int nblocks, int phase, hipStream_t s);
+                             int nblocks, int phase, int wide, hipStream_t s);
 // dense [sys][5][nk][nj][ni] (device) <-> pitched fields; to_dev = 1: dense -> fields
 // fortran = 1: the host array is the reference's u(5, i, j, k) [parts fastest]
 hipError_t launch_pack(const Geom& G, double* dense, double* const fields[5], int nsys, int to_dev, hipStream_t s,

@@ -1139,17 +1139,21 @@ __host__ __device__ inline int64_t fill_items(const Geom& G, int nsys, int phase
   return G.plane * 2 * 5 * nsys;
 }
 
-template <int PHASE>
-__device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
+// A work item's index is split with the index type I: uint32_t whenever the
+// pass has fewer than 2^32 items (every grid whose fields are below 2 GiB),
+// int64_t otherwise; 64-bit division by a runtime divisor costs several times
+// the 32-bit one, and made the theta pass bound by its index arithmetic.
+template <int PHASE, typename I>
+__device__ __forceinline__ void fill_item(const FillArgs& a, I e) {
   const Geom& G = a.G;
   if constexpr (PHASE == 0) {
     // boundary_xL (:516-517) and boundary_xR (:525-528)
-    const int jj = (int)(e % G.nj);
-    int64_t r = e / G.nj;
-    const int kk = (int)(r % G.nk);
-    r /= G.nk;
+    const int jj = (int)(e % (I)G.nj);
+    I r = e / (I)G.nj;
+    const int kk = (int)(r % (I)G.nk);
+    r /= (I)G.nk;
     const int c = (int)(r % 5);
-    const int64_t sys = (r / 5) * G.sys_stride;
+    const int64_t sys = (int64_t)(r / 5) * G.sys_stride;
     gdouble* u = gptr(a.u[c]) + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
     u[0] = 3.0 * u[1] - 3.0 * u[2] + u[3];
     if (G.bc_inflow) {
@@ -1162,14 +1166,14 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
     // boundary_yL (:542-555) and boundary_yR (:565-578): whole r-lines,
     // rows -gg <- gg+1 and Ny+gg <- Ny-gg-1, taken at k_target = mod(k+Nz/2, Nz-1)
     const int nrow = G.g + 1;
-    const int ii = (int)(e % G.ni);
-    int64_t r = e / G.ni;
-    const int row = (int)(r % (2 * nrow));
-    r /= 2 * nrow;
-    const int kk = (int)(r % G.nk);
-    r /= G.nk;
+    const int ii = (int)(e % (I)G.ni);
+    I r = e / (I)G.ni;
+    const int row = (int)(r % (I)(2 * nrow));
+    r /= (I)(2 * nrow);
+    const int kk = (int)(r % (I)G.nk);
+    r /= (I)G.nk;
     const int c = (int)(r % 5);
-    const int64_t sys = (r / 5) * G.sys_stride;
+    const int64_t sys = (int64_t)(r / 5) * G.sys_stride;
     const int kt = (kk + G.Nz / 2) % (G.Nz - 1);
     int jd, js;
     if (row < nrow) {
@@ -1184,12 +1188,12 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
     u[(int64_t)kk * G.plane + (int64_t)jd * G.ld + ii] = u[(int64_t)kt * G.plane + (int64_t)js * G.ld + ii];
   } else {
     // boundary_zL (:587) and boundary_zR (:596)
-    const int64_t t = e % G.plane;
-    int64_t r = e / G.plane;
+    const int64_t t = (int64_t)(e % (I)G.plane);
+    I r = e / (I)G.plane;
     const int side = (int)(r % 2);
     r /= 2;
     const int c = (int)(r % 5);
-    const int64_t sys = (r / 5) * G.sys_stride;
+    const int64_t sys = (int64_t)(r / 5) * G.sys_stride;
     gdouble* u = gptr(a.u[c]) + sys;
     if (side == 0)
       u[t] = u[(int64_t)(G.Nz - 1) * G.plane + t];
@@ -1200,19 +1204,19 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, int64_t e) {
 
 // The three passes run in order (xL/xR, yL/yR, zL/zR; lines_method_lib.f90:313-330);
 // each reads only points it does not write.
-template <int PHASE>
+template <int PHASE, typename I>
 __global__ __launch_bounds__(256) void fill_kernel(const FillArgs a, int64_t n) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < n) fill_item<PHASE>(a, e);
+  if (e < n) fill_item<PHASE, I>(a, (I)e);
 }
 
-template <int PHASE>
+template <int PHASE, typename I>
 __global__ __launch_bounds__(256) void fill_kernel_batch(const FillArgs* __restrict__ descs,
                                                          const int* __restrict__ bstart, int ngrids,
                                                          const int64_t* __restrict__ nitems) {
   const int q = find_grid(bstart, ngrids, (int)blockIdx.x);
   const int64_t e = (int64_t)(blockIdx.x - bstart[q]) * 256 + threadIdx.x;
-  if (e < nitems[q]) fill_item<PHASE>(descs[q], e);
+  if (e < nitems[q]) fill_item<PHASE, I>(descs[q], (I)e);
 }
 
 // dense [sys][5][nk][nj][ni] <-> pitched fields
@@ -1382,28 +1386,40 @@ hipError_t launch_stage(int mode, const StageArgs& a0, hipStream_t s) {
   }
 }
 
-hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase, int nsys) {
-  const int64_t n = fill_items(a.G, nsys, phase);
+template <typename I>
+hipError_t launch_fill_i(const FillArgs& a, hipStream_t s, int phase, int64_t n) {
   const dim3 grid((unsigned)((n + 255) / 256));
   if (phase == 0)
-    hipLaunchKernelGGL(fill_kernel<0>, grid, dim3(256), 0, s, a, n);
+    hipLaunchKernelGGL((fill_kernel<0, I>), grid, dim3(256), 0, s, a, n);
   else if (phase == 1)
-    hipLaunchKernelGGL(fill_kernel<1>, grid, dim3(256), 0, s, a, n);
+    hipLaunchKernelGGL((fill_kernel<1, I>), grid, dim3(256), 0, s, a, n);
   else
-    hipLaunchKernelGGL(fill_kernel<2>, grid, dim3(256), 0, s, a, n);
+    hipLaunchKernelGGL((fill_kernel<2, I>), grid, dim3(256), 0, s, a, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase, int nsys) {
+  const int64_t n = fill_items(a.G, nsys, phase);
+  return n < (int64_t{1} << 32) ? launch_fill_i<uint32_t>(a, s, phase, n) : launch_fill_i<int64_t>(a, s, phase, n);
+}
+
+template <typename I>
+hipError_t launch_fill_batch_i(const FillArgs* descs, const int* bstart, int ngrids, const int64_t* nitems,
+                               int nblocks, int phase, hipStream_t s) {
+  const dim3 grid((unsigned)nblocks);
+  if (phase == 0)
+    hipLaunchKernelGGL((fill_kernel_batch<0, I>), grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
+  else if (phase == 1)
+    hipLaunchKernelGGL((fill_kernel_batch<1, I>), grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
+  else
+    hipLaunchKernelGGL((fill_kernel_batch<2, I>), grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_batch(const FillArgs* descs, const int* bstart, int ngrids, const int64_t* nitems,
-                             int nblocks, int phase, hipStream_t s) {
-  const dim3 grid((unsigned)nblocks);
-  if (phase == 0)
-    hipLaunchKernelGGL(fill_kernel_batch<0>, grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
-  else if (phase == 1)
-    hipLaunchKernelGGL(fill_kernel_batch<1>, grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
-  else
-    hipLaunchKernelGGL(fill_kernel_batch<2>, grid, dim3(256), 0, s, descs, bstart, ngrids, nitems);
-  return hipGetLastError();
+                             int nblocks, int phase, int wide, hipStream_t s) {
+  return wide ? launch_fill_batch_i<int64_t>(descs, bstart, ngrids, nitems, nblocks, phase, s)
+              : launch_fill_batch_i<uint32_t>(descs, bstart, ngrids, nitems, nblocks, phase, s);
 }
 
 int64_t fill_work_items(const Geom& G, int nsys, int phase) { return fill_items(G, nsys, phase); }
