@@ -152,3 +152,14 @@ def test_fill_rule_phi_chunks():
     assert fill_kc([grid(256, 16, 32)]) == 8  # nothing fills: the shortest
     assert fill_kc([big] * 4, candidates=(8, 16, 32, 64)) == 64
     assert fill_kc([grid(24, 8, 16)], min_workgroups=1) == 16  # 17 planes: a 32 chunk would leave < 1
+
+
+def test_poisson_requests_take_a_grid_list():
+    """A stream of given grids (the small-request serving workload): the
+    requests carry those grids in order, with the stream's arrivals and steps."""
+    specs = [MemberSpec(256, 16, 32, 0.1 * q, 1.0, 1.8, 60.0, 15.0, 1.5, {(0, 0): 1.0}, q) for q in range(5)]
+    reqs = poisson_requests(5, specs=specs, seed=9)
+    assert [r.spec for r in reqs] == specs
+    base = poisson_requests(5, seed=9)
+    assert [r.arrival for r in reqs] == [r.arrival for r in base]
+    assert [r.nsteps for r in reqs] == [r.nsteps for r in base]
