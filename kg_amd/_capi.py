@@ -127,7 +127,7 @@ def load() -> C.CDLL:
     # loaded (same soname), and device pointers and streams are shared.
     try:
         import torch  # noqa: F401
-    except ImportError:
+    except (ImportError, OSError):  # absent, or present without its libraries
         pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
