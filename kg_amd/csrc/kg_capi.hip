@@ -773,6 +773,7 @@ int aux_count(int scheme) { return scheme == KG_PIRK2 ? 1 : scheme == KG_PIRK3 ?
 bool same_grid(const kg_desc& a, const kg_desc& b) {
   return a.nvars == b.nvars && a.g_pts == b.g_pts && a.Nx == b.Nx && a.Ny == b.Ny && a.Nz == b.Nz &&
          a.scheme == b.scheme && a.metric == b.metric && a.bc_rmax == b.bc_rmax && a.is_complex == b.is_complex &&
+         a.op_set == b.op_set &&
          a.xmin == b.xmin && a.xmax == b.xmax && a.dt == b.dt && a.a == b.a && a.M_bh == b.M_bh && a.m == b.m;
 }
 }  // namespace

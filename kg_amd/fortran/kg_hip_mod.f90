@@ -229,7 +229,14 @@ contains
       print*, 'try with: outflow or inflow'
       stop
     end if
-    d%is_complex = merge(1, 0, is_complex)
+    ! u is complex(8) in the reference (lines_method_lib.f90:27) and the
+    ! upload/download procedures below pass it as such, so a real handle
+    ! would mix the re/im parts of neighbouring values: refused
+    if (.not. is_complex) then
+      print*, 'kg_lines_method_constructor: the state u is complex(8); is_complex must be .true.'
+      stop
+    end if
+    d%is_complex = 1
     d%op_set = KG_OPS_REF2
     if (present(op_set)) d%op_set = int(op_set, c_int32_t)
     d%xmin = xmin; d%xmax = xmax; d%tmin = tmin; d%CFL = CFL; d%dt = 0.0d0

@@ -46,3 +46,18 @@ def test_checkpoint_of_another_grid_is_refused(tmp_path):
         b.read_from_check_point(tmp_path / "a.bin")
     with pytest.raises(kg_amd.KGError):
         b.read_from_check_point(tmp_path / "missing.bin")
+
+
+def test_checkpoint_of_another_operator_set_is_refused(tmp_path):
+    # same grid and scheme, the other stencil set: restarting would continue
+    # with a different right-hand side
+    kw = dict(Nx=20, Ny=12, Nz=14, xmin=1.9, xmax=12.0, a=0.6, m=0.3, boundary_type_rmax="inflow", scheme="rk4")
+    a = kg_amd.LinesMethod(operators="fd4", **kw)
+    a.check_point(tmp_path / "fd4.bin")
+    b = kg_amd.LinesMethod(operators="ref2", **kw)
+    with pytest.raises(kg_amd.KGError):
+        b.read_from_check_point(tmp_path / "fd4.bin")
+    b.check_point(tmp_path / "ref2.bin")
+    with pytest.raises(kg_amd.KGError):
+        a.read_from_check_point(tmp_path / "ref2.bin")
+    a.read_from_check_point(tmp_path / "fd4.bin")
