@@ -25,6 +25,10 @@ SCHEMES = {"pirk2": PIRK2, "pirk3": PIRK3, "rk3": RK3, "rk4": RK4}
 MINKOWSKI, KERR_SCHILD, MKS, SCHW_EF = 0, 1, 2, 3
 METRICS = {"Minkowski": MINKOWSKI, "Kerr_Schild": KERR_SCHILD, "MKS": MKS, "Schw_EF": SCHW_EF}
 OUTFLOW, INFLOW = 0, 1
+# deliberate mistakes the oracle can plant (kg_oracle.h KGO_PLANT_*), for the
+# tests' mutation checks
+PLANTS = {"drop_pi_dxbeta": 1, "swap_phi_flux": 2, "psi2_alpha_pi": 3, "drop_psi1_dxbeta": 4,
+          "gdn13_no_sin2": 5, "advect_pi": 6, "mass_sign": 7}
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
@@ -59,6 +63,8 @@ def _load():
     lib.kgo_advec_x_4_raw.argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp, _dp]
     lib.kgo_set_order.argtypes = [C.c_void_p, C.c_int]
     lib.kgo_set_order.restype = C.c_int
+    lib.kgo_plant.argtypes = [C.c_void_p, C.c_int]
+    lib.kgo_plant.restype = C.c_int
     lib.kgo_threads.restype = C.c_int
     lib.kgo_threads.argtypes = []
     for nm in ("kgo_rhs_phi", "kgo_rhs_psi1", "kgo_rhs_psi2", "kgo_rhs_psi3", "kgo_rhs_pi", "kgo_rhs_all"):
@@ -150,6 +156,11 @@ class Sim:
         """Operator set of the RHS: 2 (the reference's live stencils) or 4 (its fd4 set)."""
         if lib().kgo_set_order(self._h, order) != 0:
             raise ValueError("order must be 2 or 4")
+
+    def plant(self, code):
+        """Plant deliberate mistake `code` (a PLANTS value; 0 removes it) for a mutation check."""
+        if lib().kgo_plant(self._h, int(code)) != 0:
+            raise ValueError("unknown planted mistake %r" % (code,))
 
     def step(self, scheme, nsteps=1):
         lib().kgo_step(self._h, SCHEMES[scheme] if isinstance(scheme, str) else scheme, nsteps)

@@ -95,6 +95,7 @@ double kgo_reference_dt(int Nx, int Ny, int Nz, int g, double xmin, double xmax,
 #define GUP(mu, nu) gup[(mu) * 4 + (nu)]
 
 static void metric_point(const kgo_sim* s, double x, double y, double* gdn) {
+  const double s2_13 = (s->plant == KGO_PLANT_GDN13_NO_SIN2) ? 1.0 : sin(y) * sin(y);
   const double M = s->M_bh, a = s->a;
   for (int q = 0; q < 16; ++q) gdn[q] = 0.0;
   if (s->metric == KGO_METRIC_MINKOWSKI) {           /* geometry.f90:104-110 */
@@ -115,7 +116,7 @@ static void metric_point(const kgo_sim* s, double x, double y, double* gdn) {
     GDN(1, 0) = GDN(0, 1);
     GDN(1, 1) = (1.0 + 2.0 * M * x / (x * x + a2 * c2));
     GDN(1, 2) = 0.0;
-    GDN(1, 3) = (-a * (1.0 + 2.0 * M * x / (x * x + a2 * c2)) * s2);
+    GDN(1, 3) = (-a * (1.0 + 2.0 * M * x / (x * x + a2 * c2)) * s2_13);
     GDN(2, 2) = (x * x + a2 * c2);
     GDN(3, 0) = GDN(0, 3);
     GDN(3, 1) = GDN(1, 3);
@@ -453,6 +454,13 @@ static void dx_beta_build(kgo_sim* s) {
   }
 }
 
+int kgo_plant(kgo_sim* s, int code) {
+  if (code < 0 || code >= KGO_PLANT_NUM) return -1;
+  s->plant = code;
+  metric_build(s);
+  return 0;
+}
+
 int kgo_set_order(kgo_sim* s, int order) {
   if (order != 2 && order != 4) return -1;
   s->order = order;
@@ -562,7 +570,8 @@ void kgo_rhs_psi1(kgo_sim* s, double* rhs, const double* vars) {
     for (int jj = 0; jj < s->nj; ++jj)
       for (int ii = 0; ii < s->ni; ++ii) {
         int64_t p = IDX(s, ii, jj, kk), p2 = IDX2(s, ii, jj);
-        rhs[p] = t1[p] + T2(s->beta_r) * t2[p] + T2(s->dx_beta) * v2[p];
+        rhs[p] = (s->plant == KGO_PLANT_DROP_PSI1_DXBETA) ? t1[p] + T2(s->beta_r) * t2[p]
+                                                          : t1[p] + T2(s->beta_r) * t2[p] + T2(s->dx_beta) * v2[p];
       }
 }
 
@@ -581,6 +590,10 @@ static void flux_F(kgo_sim* s, const double* vars, double* F) {
 void kgo_rhs_psi2(kgo_sim* s, double* rhs, const double* vars) {
   /* :700  rhs = dx2(alpha*vars(5) + beta*vars(2)) */
   flux_F(s, vars, s->tmp[0]);
+  if (s->plant == KGO_PLANT_PSI2_ALPHA_PI) {
+    const double* v5 = FIELD(vars, 4);
+    for (int64_t p = 0; p < s->npts; ++p) s->tmp[0][p] = s->alpha[p % ((int64_t)s->ni * s->nj)] * v5[p];
+  }
   op_dth(s, s->tmp[0], rhs);
 }
 
@@ -607,13 +620,14 @@ void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars) {
         double al = T2(s->alpha), sg = T2(s->sqrtg);
         if (!ef) {
           fr[p] = al * sg * (T2(s->g_rr) * v2[p] + T2(s->g_rp) * v4[p]);
-          fp[p] = al * sg * (T2(s->g_rp) * v2[p] + T2(s->g_pp) * v4[p]);
+          fp[p] = (s->plant == KGO_PLANT_SWAP_PHI_FLUX) ? al * sg * (T2(s->g_pp) * v2[p] + T2(s->g_rp) * v4[p])
+                                                        : al * sg * (T2(s->g_rp) * v2[p] + T2(s->g_pp) * v4[p]);
         } else {
           fr[p] = al * sg * T2(s->g_rr) * v2[p];
           fp[p] = al * T2(s->g_pp) * v4[p];
         }
         ft[p] = al * sg * T2(s->g_tt) * v3[p];
-        ps[p] = v5[p] * sg;
+        ps[p] = (s->plant == KGO_PLANT_ADVECT_PI) ? v5[p] : v5[p] * sg;
       }
   op_dr(s, fr, d1);
   op_dth(s, ft, d2);
@@ -627,8 +641,9 @@ void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars) {
         int64_t p = IDX(s, ii, jj, kk), p2 = IDX2(s, ii, jj);
         double sg = T2(s->sqrtg);
         double t3 = ef ? d3[p] : d3[p] / sg;
-        rhs[p] = d1[p] / sg + d2[p] / sg + t3 + v5[p] * T2(s->dx_beta) + T2(s->beta_r) * ad[p] / sg -
-                 T2(s->alpha) * m2 * v1[p];
+        const double pdb = (s->plant == KGO_PLANT_DROP_PI_DXBETA) ? 0.0 : v5[p] * T2(s->dx_beta);
+        const double mass = (s->plant == KGO_PLANT_MASS_SIGN) ? -(T2(s->alpha) * m2 * v1[p]) : T2(s->alpha) * m2 * v1[p];
+        rhs[p] = d1[p] / sg + d2[p] / sg + t3 + pdb + T2(s->beta_r) * ad[p] / sg - mass;
       }
 }
 

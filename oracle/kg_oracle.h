@@ -63,6 +63,7 @@ typedef struct kgo_sim {
   /* time stepping */
   int bc_rmax;
   int order;           /* operator set: 2 = the reference's live stencils, 4 = its fd4 set */
+  int plant;           /* KGO_PLANT_*: a deliberate mistake for the tests' mutation checks (0: none) */
   double t, dt, CFL;
   /* state: 5 fields each, npts doubles per field */
   double *u, *u_p, *u_stage, *u_stage_2, *rhs_u, *k1, *k2, *k3, *k4;
@@ -109,6 +110,25 @@ void kgo_advec_x_4_raw(int ni, int nj, int nk, double h, const double* f, const 
  * first derivative of lines_method_lib.f90:671-753 uses the fd4 set, and so
  * does the metric's d_r beta^r.  Returns 0, or -1 for another order. */
 int kgo_set_order(kgo_sim* s, int order);
+
+/* Deliberate mistakes, planted so the tests can show that their pins catch
+ * them (mutation checks; 0 = the restatement as written).  Each is a
+ * plausible slip in transcribing lines_method_lib.f90:671-753 or
+ * geometry.f90:112-160. */
+enum {
+  KGO_PLANT_NONE = 0,
+  KGO_PLANT_DROP_PI_DXBETA = 1,    /* R_Pi without vars(5)*dx_beta (:735) */
+  KGO_PLANT_SWAP_PHI_FLUX = 2,     /* phi flux with gamma^{r phi} and gamma^{phi phi} swapped (:731) */
+  KGO_PLANT_PSI2_ALPHA_PI = 3,     /* R_psi_theta = D_theta(alpha Pi), beta psi_r dropped (:700) */
+  KGO_PLANT_DROP_PSI1_DXBETA = 4,  /* R_psi_r without dx_beta*vars(2) (:689) */
+  KGO_PLANT_GDN13_NO_SIN2 = 5,     /* g_dn(1,3) without its sin^2(theta) factor (geometry.f90:130) */
+  KGO_PLANT_ADVECT_PI = 6,         /* A_r of Pi instead of Pi*sqrtg in R_Pi (:733) */
+  KGO_PLANT_MASS_SIGN = 7,         /* + alpha m^2 phi in R_Pi (:736) */
+  KGO_PLANT_NUM = 8
+};
+/* Selects a planted mistake (rebuilding the metric tables); returns 0, or -1
+ * for an unknown code. */
+int kgo_plant(kgo_sim* s, int code);
 
 /* RHS pieces (lines_method_lib.f90:671-753). vars = 5-field state. */
 void kgo_rhs_phi(kgo_sim* s, double* rhs, const double* vars);
