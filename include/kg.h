@@ -154,6 +154,23 @@ kg_status kg_step(kg_handle* h, int32_t nsteps);
  * including its PIRK stage arrays, equals stepping it alone.
  * KG_ERR_UNSUPPORTED for mixed schemes, operator sets or metric forms. */
 kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
+/* kg_batch_step without allocation.  No counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and ordering as kg_batch_step, but the launch descriptors go to a
+ * device array of the library itself, 4 MiB per GPU (held until the process
+ * ends; its address is resolved at kg_create), instead of a buffer the lead
+ * handle grows; the call makes no hipMalloc/hipFree and creates no event.
+ * The descriptors take 2 x (launches per step) x n x 488 bytes plus small
+ * prefix arrays (PIRK2: 4 launches per step, about 1050 members at most;
+ * RK4: 5, about 840; RK3 and PIRK3: 6, about 700): a call that needs more
+ * returns KG_ERR_INVALID.
+ * Every member's stage buffers must exist (kg_create makes those of its
+ * desc.scheme).  Kernel timing (kg_profile_enable) is not recorded.  All
+ * calls on one GPU share the array and the last call's descriptors stay in
+ * it: calls on one GPU that overlap (from other threads, or on other
+ * streams than an earlier call's still-running lead stream) race. */
+kg_status kg_batch_step_noalloc(kg_handle* const* hs, int32_t n, int32_t nsteps);
 /* waits for the handle's stream */
 kg_status kg_sync(kg_handle* h);
 

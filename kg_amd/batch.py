@@ -27,15 +27,18 @@ ANGULAR = [(16, 32), (32, 64), (64, 128)]
 DR = 58.2 / 256  # radial spacing of Nr = 256 on r in [1.8, 60]
 
 
-def batch_step(members, nsteps: int = 1) -> None:
+def batch_step(members, nsteps: int = 1, noalloc: bool = False) -> None:
     """Advance every LinesMethod in `members` by nsteps steps of their common
     scheme (rk4, rk3, pirk2 or pirk3) in batched launches: one per stage
-    kernel and one per ghost-fill pass over all members."""
+    kernel and one per ghost-fill pass over all members.  noalloc: through
+    kg_batch_step_noalloc, which keeps its launch descriptors in the
+    library's own fixed device array (no allocation in the call)."""
     if not members:
         return
     lib = capi.load()
     arr = (C.c_void_p * len(members))(*[m._h for m in members])
-    st = lib.kg_batch_step(arr, len(members), int(nsteps))
+    fn = lib.kg_batch_step_noalloc if noalloc else lib.kg_batch_step
+    st = fn(arr, len(members), int(nsteps))
     capi.check(st, members[0]._h)
 
 

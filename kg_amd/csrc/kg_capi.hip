@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -76,8 +77,8 @@ struct kg_handle {
   int64_t prof_n[2] = {0, 0};
   void* batch_dev = nullptr;  // descriptors of batched launches led by this handle
   size_t batch_cap = 0;
-  mutable void* scratch = nullptr;  // reductions of the diagnostics, kept between calls
-  mutable size_t scratch_bytes = 0;
+  mutable void* reduce_buf = nullptr;  // reductions of the diagnostics, kept between calls
+  mutable size_t reduce_bytes = 0;
   hipEvent_t batch_done = nullptr;
   std::vector<RecOp>* rec = nullptr;  // when set, stage() and fill() record instead of launching
 };
@@ -428,7 +429,53 @@ kg_status xfer(kg_handle* h, const State5& st, double* host, int32_t layout, int
 // both values of its buffer and stale-Pi parities, then replayed for all
 // members together: member q's part of launch o is exactly its own launch o.
 // ---------------------------------------------------------------------------
-kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
+namespace {
+// Descriptor block of kg_batch_step_noalloc: device memory of the library
+// itself (one copy per GPU, held until the process ends), so that call makes
+// no allocation.  Host code reaches it only through the address resolved per
+// device at set-up (kg_create, with that device current).
+constexpr size_t kBatchScratchBytes = size_t(4) << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) char scratch[kBatchScratchBytes];
+
+namespace {
+hipError_t resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_dev[device] = p;
+  return hipSuccess;
+}
+
+void* scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_dev[device];
+}
+
+// the stage buffers a step of `scheme` uses already exist
+bool buffers_ready(const kg_handle* h, int scheme) {
+  switch (scheme) {
+    case KG_RK4: return h->Ua.f[0] && h->Ub.f[0] && h->A.f[0];
+    case KG_RK3: return h->Ua.f[0] && h->Ub.f[0];
+    case KG_PIRK2: return h->S.f[0] != nullptr;
+    case KG_PIRK3: return h->S.f[0] && h->S2.f[0];
+  }
+  return false;
+}
+}  // namespace
+
+// noalloc: the descriptors go to the library's own device array instead of
+// the lead handle's growable buffer, no buffer or event is created and the
+// kernel-timing events are not recorded.
+kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
   kg_handle* lead = hs[0];
   const int ef = lead->G.metric_ef;
   const int scheme = lead->desc.scheme;
@@ -441,7 +488,12 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
     if (h->device != lead->device) return fail(lead, KG_ERR_INVALID, "batch: handles on different devices");
     for (int p = 0; p < q; ++p)
       if (hs[p] == h) return fail(lead, KG_ERR_INVALID, "batch: handle %d repeated", q);
-    KG_TRY(ensure_buffers(h, scheme));
+    if (noalloc) {
+      if (!buffers_ready(h, scheme))
+        return fail(lead, KG_ERR_INVALID, "batch without allocation: handle %d has no stage buffers for its scheme", q);
+    } else {
+      KG_TRY(ensure_buffers(h, scheme));
+    }
     if (h != lead) KG_HIP(lead, hipStreamSynchronize(h->stream));
   }
   // two consecutive steps of every member: version v is the step taken after
@@ -517,15 +569,25 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
   memcpy(host.data() + o_ts, tstart.data(), tstart.size() * sizeof(int));
   memcpy(host.data() + o_bs, bstart.data(), bstart.size() * sizeof(int));
   memcpy(host.data() + o_ni, nitems.data(), nitems.size() * sizeof(int64_t));
-  if (lead->batch_cap < bytes) {
-    KG_HIP(lead, hipStreamSynchronize(lead->stream));
-    if (lead->batch_dev) KG_HIP(lead, hipFree(lead->batch_dev));
-    lead->batch_dev = nullptr;
-    lead->batch_cap = 0;
-    KG_HIP(lead, hipMalloc(&lead->batch_dev, bytes));
-    lead->batch_cap = bytes;
+  char* dev = nullptr;
+  if (noalloc) {
+    if (bytes > kBatchScratchBytes)
+      return fail(lead, KG_ERR_INVALID,
+                  "batch without allocation: %zu bytes of launch descriptors exceed its %zu-byte device array "
+                  "(fewer members per call)", bytes, kBatchScratchBytes);
+    dev = (char*)scratch_of(lead->device);
+    if (!dev) return fail(lead, KG_ERR_INVALID, "batch without allocation: device %d not set up", lead->device);
+  } else {
+    if (lead->batch_cap < bytes) {
+      KG_HIP(lead, hipStreamSynchronize(lead->stream));
+      if (lead->batch_dev) KG_HIP(lead, hipFree(lead->batch_dev));
+      lead->batch_dev = nullptr;
+      lead->batch_cap = 0;
+      KG_HIP(lead, hipMalloc(&lead->batch_dev, bytes));
+      lead->batch_cap = bytes;
+    }
+    dev = (char*)lead->batch_dev;
   }
-  char* dev = (char*)lead->batch_dev;
   KG_HIP(lead, hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, lead->stream));
   const StageArgs* dsd = (const StageArgs*)dev;
   const FillArgs* dfd = (const FillArgs*)(dev + o_fd);
@@ -537,8 +599,9 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
     for (size_t o = 0; o < nops; ++o) {
       const size_t vo = (size_t)v * nops + o;
       const int kind = rec[0][o].kind;
+      const bool prof = lead->prof && !noalloc;
       hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (lead->prof) {
+      if (prof) {
         e0 = next_event(lead);
         e1 = next_event(lead);
         if (!e0 || !e1) return fail(lead, KG_ERR_HIP, "hipEventCreate failed");
@@ -552,14 +615,15 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps) {
           KG_HIP(lead, launch_fill_batch(dfd + vo * n, dbs + ph * (n + 1), n, dni + ph * n,
                                          bstart[ph * (n + 1) + n], ph, wide[ph], lead->stream));
       }
-      if (lead->prof) {
+      if (prof) {
         KG_HIP(lead, hipEventRecord(e1, lead->stream));
         lead->ev_marks.push_back({kind, lead->ev_used - 2});
       }
     }
   }
   // bookkeeping and ordering: later work on any member waits for the batch
-  if (!lead->batch_done) KG_HIP(lead, hipEventCreateWithFlags(&lead->batch_done, hipEventDisableTiming));
+  // (the event is made at kg_create)
+  if (!lead->batch_done) return fail(lead, KG_ERR_INVALID, "internal: batch event missing");
   KG_HIP(lead, hipEventRecord(lead->batch_done, lead->stream));
   for (int q = 0; q < n; ++q) {
     kg_handle* h = hs[q];
@@ -645,6 +709,15 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
     delete h;
     return KG_ERR_HIP;
   }
+  // set-up of the batched paths on this device: the ordering event, and the
+  // address of the no-allocation descriptor array (once per device)
+  e = hipEventCreateWithFlags(&h->batch_done, hipEventDisableTiming);
+  if (e == hipSuccess) e = resolve_scratch(h->device);
+  if (e != hipSuccess) {
+    fail(nullptr, KG_ERR_HIP, "batch set-up failed: %s", hipGetErrorString(e));
+    kg_destroy(h);
+    return KG_ERR_HIP;
+  }
   std::vector<double> tab((size_t)kNumTables * G.nj * G.ld);
   build_tables(h->hg, d->metric, d->a, d->M_bh, d->m, G.fd4, tab.data());
   e = hipMalloc(&h->d_tab, tab.size() * sizeof(double));
@@ -676,7 +749,7 @@ kg_status kg_destroy(kg_handle* h) {
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->batch_dev) (void)hipFree(h->batch_dev);
-  if (h->scratch) (void)hipFree(h->scratch);
+  if (h->reduce_buf) (void)hipFree(h->reduce_buf);
   if (h->batch_done) (void)hipEventDestroy(h->batch_done);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -866,7 +939,13 @@ kg_status kg_step(kg_handle* h, int32_t nsteps) {
 kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps) {
   if (!hs || n < 1 || !hs[0]) return fail(nullptr, KG_ERR_INVALID, "batch: need at least one handle");
   if (nsteps < 0) return fail(hs[0], KG_ERR_INVALID, "nsteps < 0");
-  return batch_step(hs, n, nsteps);
+  return batch_step(hs, n, nsteps, false);
+}
+
+kg_status kg_batch_step_noalloc(kg_handle* const* hs, int32_t n, int32_t nsteps) {
+  if (!hs || n < 1 || !hs[0]) return fail(nullptr, KG_ERR_INVALID, "batch: need at least one handle");
+  if (nsteps < 0) return fail(hs[0], KG_ERR_INVALID, "nsteps < 0");
+  return batch_step(hs, n, nsteps, true);
 }
 
 kg_status kg_sync(kg_handle* h) {
@@ -919,18 +998,18 @@ kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t*
 
 namespace kg {
 hipError_t handle_scratch(const kg_handle* h, size_t bytes, void** out) {
-  if (h->scratch_bytes < bytes) {
-    if (h->scratch) {
+  if (h->reduce_bytes < bytes) {
+    if (h->reduce_buf) {
       (void)hipStreamSynchronize(h->stream);
-      (void)hipFree(h->scratch);
-      h->scratch = nullptr;
-      h->scratch_bytes = 0;
+      (void)hipFree(h->reduce_buf);
+      h->reduce_buf = nullptr;
+      h->reduce_bytes = 0;
     }
-    hipError_t e = hipMalloc(&h->scratch, bytes);
+    hipError_t e = hipMalloc(&h->reduce_buf, bytes);
     if (e != hipSuccess) return e;
-    h->scratch_bytes = bytes;
+    h->reduce_bytes = bytes;
   }
-  *out = h->scratch;
+  *out = h->reduce_buf;
   return hipSuccess;
 }
 
