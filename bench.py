@@ -258,6 +258,35 @@ def serving(nreq, tuning, operators, batch_kc=None, specs=None, workload=None):
                                      "chunks of 50 RK4 steps, final fields + l<=8, 512-k spectrum (split-BF16)")}
 
 
+PROBE_KINDS = {0: ("read", 1, 0), 1: ("write", 0, 1), 2: ("copy", 1, 1), 3: ("3 reads + 1 write", 3, 1),
+               4: ("copy, non-temporal", 1, 1), 5: ("3 reads + 1 write, non-temporal", 3, 1)}
+
+
+def hbm_probe(n=192 * 2 ** 20, reps=10, blocks=(1024, 2048, 4096, 8192, 16384)):
+    """What plain streaming kernels move on this GPU (kg_hbm_probe): FP64
+    arrays of n doubles (1.5 GiB each), 16-byte accesses, grid-stride loops,
+    `reps` timed launches per launch shape; TB/s = bytes moved / average
+    launch time."""
+    import ctypes as C
+    from kg_amd import _capi as capi
+
+    lib = capi.load()
+    out = {}
+    for kind, (name, nr, nw) in PROBE_KINDS.items():
+        per = {}
+        for nb in blocks:
+            ms = C.c_double()
+            st = lib.kg_hbm_probe(0, kind, n, nb, reps, C.byref(ms))
+            if st != 0:
+                raise RuntimeError(lib.kg_probe_last_error().decode())
+            per[nb] = n * 8 * (nr + nw) / (ms.value * 1e-3) / 1e12
+            print("bench: probe %s, %d workgroups: %.3f TB/s" % (name, nb, per[nb]), file=sys.stderr, flush=True)
+        best = max(per, key=per.get)
+        out[name] = {"best_TB_per_s": per[best], "best_workgroups": best, "frac_of_peak": per[best] * 1e3 / HBM_PEAK_GBS,
+                     "TB_per_s_by_workgroups": {str(k): v for k, v in per.items()}}
+    return {"array_doubles": n, "reps": reps, "threads_per_workgroup": 256, "kinds": out}
+
+
 KERNELS = {"ref2": "stage_kernel (RHS + RK4 stage combine, the reference's 2nd-order stencils)",
            "fd4": "stage4_kernel (RHS + RK4 stage combine, the 4th-order stencil set)"}
 
@@ -373,6 +402,8 @@ def main():
     ap.add_argument("--ic", type=int, default=512,
                     help="Cartesian points per axis of the random_shell initial data to time (0: skip)")
     ap.add_argument("--ic-only", action="store_true", help="time only the initial data and exit")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="measure what plain streaming kernels move (read, write, copy, 3 reads + 1 write) and exit")
     ap.add_argument("--serve-small", type=int, default=64,
                     help="requests of the population's smallest class through the serving loop (0: skip)")
     ap.add_argument("--serve", type=int, default=8,
@@ -388,6 +419,9 @@ def main():
     assert torch.cuda.is_available(), "no GPU visible"
     if args.ic_only:
         print(json.dumps({"initial_data": initial_data(args.ic or 512)}))
+        return
+    if args.hbm_probe:
+        print(json.dumps({"hbm_probe": hbm_probe()}))
         return
     head = plain(args.operators, args.steps, args.warmup, args.kc, args.tj)
     order = "4th-order" if args.operators == "fd4" else "the reference's 2nd-order"

@@ -316,6 +316,17 @@ kg_status kg_ic_random_field(int32_t device, int32_t ncart, double r_max, double
                              const double* uniforms, double* out);
 const char* kg_ic_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * HBM bandwidth probe (measurement only; no reference counterpart): the
+ * average duration of `reps` launches of a streaming kernel over FP64 arrays
+ * of n doubles (n even) on `device`, `blocks` workgroups of 256 threads.
+ * kind 0: read (n x 8 B); 1: write (n x 8 B); 2: copy (read + write);
+ * 3: d = a + b + c (three reads, one write); 4, 5: copy and that mix with
+ * non-temporal loads and stores.  Allocates and frees its four arrays.
+ * ------------------------------------------------------------------------ */
+kg_status kg_hbm_probe(int32_t device, int32_t kind, int64_t n, int32_t blocks, int32_t reps, double* ms_per_launch);
+const char* kg_probe_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,8 @@ SIGNATURES = {
     "kg_initial_data": (C.c_int32, [_vp, C.POINTER(KGICDesc), _dp]),
     "kg_ic_random_field": (C.c_int32, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, _dp, _dp]),
     "kg_ic_last_error": (C.c_char_p, []),
+    "kg_hbm_probe": (C.c_int32, [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _dp]),
+    "kg_probe_last_error": (C.c_char_p, []),
 }
 
 
