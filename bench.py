@@ -4,7 +4,8 @@ Workload (the full single-GPU configuration): massive (m = 0.1) Klein-Gordon
 field on Schwarzschild in Kerr-Schild coordinates (M = 1), Nr = 2048 on
 r in [1.8, 200], Ntheta = 128, Nphi = 256 (67.1 M points x 5 fields), classic
 RK4 with the 4th-order stencil set of tools/finite_differences.f90 (x_4, y_4,
-z_4, advec_x_4), dt from CFL 0.25.  Initial phi is a Gaussian shell (r0 = 30,
+z_4, advec_x_4), dt from CFL 0.25, Sommerfeld outer boundary (--bc outflow:
+the reference's copy).  Initial phi is a Gaussian shell (r0 = 30,
 width 3) times sum_{l<=4} c_lm Y_lm with c_lm ~ N(0, 1e-3) from a seed; psi =
 its gradient, Pi = 0.  The same workload with the reference's live 2nd-order
 stencils is timed beside it (`plain_ref2`; --operators ref2 swaps the two).
@@ -46,7 +47,8 @@ B_PER_UPDATE = {"rk4": 680, "rk3": 320, "pirk2": 248, "pirk3": 416}
 PLAIN_SHAPES = [(kc, tj) for tj in (4, 8) for kc in (1, 2, 4, 8, 16)]
 STAGES = {"rk4": 4, "rk3": 3, "pirk2": 3, "pirk3": 5}
 
-CFG = dict(Nx=2048, Ny=129, Nz=257, xmin=1.8, xmax=200.0, m=0.1, M_bh=1.0, a=0.0, CFL=0.25, scheme="rk4")
+CFG = dict(Nx=2048, Ny=129, Nz=257, xmin=1.8, xmax=200.0, m=0.1, M_bh=1.0, a=0.0, CFL=0.25, scheme="rk4",
+           bc="sommerfeld")
 
 
 def cpu_baseline(operators, max_seconds=12.0):
@@ -305,7 +307,7 @@ def plain(operators, steps, warmup, kc=0, tj=8, scheme=None):
     print("bench: building initial data (%s)" % operators, file=sys.stderr, flush=True)
     lm = kg_amd.LinesMethod(Nx=CFG["Nx"], Ny=CFG["Ny"], Nz=CFG["Nz"], xmin=CFG["xmin"], xmax=CFG["xmax"],
                             CFL=CFG["CFL"], m=CFG["m"], M_bh=CFG["M_bh"], a=CFG["a"], scheme=scheme,
-                            operators=operators)
+                            operators=operators, boundary_type_rmax=CFG["bc"])
     u0 = shell_state_device(CFG["Nx"], CFG["Ny"], CFG["Nz"], 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0,
                             random_lm_coeffs(20261203))
     torch.cuda.synchronize()
@@ -402,6 +404,8 @@ def main():
     ap.add_argument("--ic", type=int, default=512,
                     help="Cartesian points per axis of the random_shell initial data to time (0: skip)")
     ap.add_argument("--ic-only", action="store_true", help="time only the initial data and exit")
+    ap.add_argument("--bc", choices=["sommerfeld", "outflow"], default="sommerfeld",
+                    help="outer boundary of the plain single-grid legs (the workload names Sommerfeld)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="measure what plain streaming kernels move (read, write, copy, 3 reads + 1 write) and exit")
     ap.add_argument("--serve-small", type=int, default=64,
@@ -409,6 +413,7 @@ def main():
     ap.add_argument("--serve", type=int, default=8,
                     help="requests of the serving loop to run beside the plain path (0: skip)")
     args = ap.parse_args()
+    CFG["bc"] = args.bc
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (a single grid lives on one MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -439,7 +444,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Gaussian shell x sum of Y_lm, l<=4)",
         "config": {"workload": "plain single-grid RK4 with %s stencils, Nr=2048 r in [1.8,200], Ntheta=128, "
-                               "Nphi=256, m=0.1, Kerr-Schild M=1, CFL 0.25" % order,
+                               "Nphi=256, m=0.1, Kerr-Schild M=1, CFL 0.25, %s outer boundary" % (order, args.bc),
                    "grid_points": CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1), "scheme": "rk4",
                    "operators": args.operators, "fields": 5, "state_finite": head["state_finite"],
                    "tuning": head["tuning"]},

@@ -52,8 +52,14 @@ enum { KG_PIRK2 = 0, KG_PIRK3 = 1, KG_RK3 = 2, KG_RK4 = 3 };
  * and Schw_EF (lines_method_lib.f90:727-748); the others are rejected at
  * kg_create with KG_ERR_UNSUPPORTED, as the reference stops there. */
 enum { KG_METRIC_MINKOWSKI = 0, KG_METRIC_KERR_SCHILD = 1, KG_METRIC_MKS = 2, KG_METRIC_SCHW_EF = 3 };
-/* boundary_type_rmax: 'outflow' / 'inflow' (lines_method_lib.f90:522-535) */
-enum { KG_BC_OUTFLOW = 0, KG_BC_INFLOW = 1 };
+/* boundary_type_rmax: 'outflow' / 'inflow' (lines_method_lib.f90:522-535),
+ * and KG_BC_SOMMERFELD, which the reference does not have: the outer column
+ * i = Nx+g is evolved instead of filled, its phi and psi with their own
+ * right-hand sides (the operator set's one-sided r rows) and its Pi with the
+ * radiative condition d_t Pi = -v (d_r Pi + Pi/r), v = alpha sqrt(gamma^rr) -
+ * beta^r the outgoing radial characteristic speed, d_r the operator set's
+ * one-sided edge row; exact for outgoing waves f(t - r)/r in flat space. */
+enum { KG_BC_OUTFLOW = 0, KG_BC_INFLOW = 1, KG_BC_SOMMERFELD = 2 };
 /* host array layouts accepted by upload/download */
 enum {
   KG_LAYOUT_FORTRAN = 0, /* u(nvars,ni,nj,nk), variable fastest: complex(8) pairs
@@ -161,9 +167,10 @@ kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
  * device array of the library itself, 4 MiB per GPU (held until the process
  * ends; its address is resolved at kg_create), instead of a buffer the lead
  * handle grows; the call makes no hipMalloc/hipFree and creates no event.
- * The descriptors take 2 x (launches per step) x n x 488 bytes plus small
- * prefix arrays (PIRK2: 4 launches per step, about 1050 members at most;
- * RK4: 5, about 840; RK3 and PIRK3: 6, about 700): a call that needs more
+ * The descriptors take 2 x (launches per step) x n x 504 bytes plus small
+ * prefix arrays (PIRK2: 4 launches per step, about 1020 members at most;
+ * RK4: 5, about 820; RK3 and PIRK3: 6, about 680; with KG_BC_SOMMERFELD a
+ * stage adds its outer-column launch: RK4 about 450): a call that needs more
  * returns KG_ERR_INVALID.
  * Every member's stage buffers must exist (kg_create makes those of its
  * desc.scheme).  Kernel timing (kg_profile_enable) is not recorded.  All

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libkg_hip.so")
 KG_OK, KG_ERR_INVALID, KG_ERR_UNSUPPORTED, KG_ERR_HIP, KG_ERR_NOMEM, KG_ERR_NO_DEVICE = range(6)
 KG_PIRK2, KG_PIRK3, KG_RK3, KG_RK4 = range(4)
 KG_METRIC_MINKOWSKI, KG_METRIC_KERR_SCHILD, KG_METRIC_MKS, KG_METRIC_SCHW_EF = range(4)
-KG_BC_OUTFLOW, KG_BC_INFLOW = range(2)
+KG_BC_OUTFLOW, KG_BC_INFLOW, KG_BC_SOMMERFELD = range(3)
 KG_LAYOUT_FORTRAN, KG_LAYOUT_SOA = range(2)
 KG_OPS_REF2, KG_OPS_FD4 = range(2)
 

@@ -40,7 +40,7 @@ struct State5 {
 // One launch of a step, recorded instead of launched (batched stepping
 // replays the recorded sequence of every member with one launch per entry).
 struct RecOp {
-  int kind;  // 0: stage kernel, 1: ghost fill (its three passes)
+  int kind;  // 0: stage kernel, 1: ghost fill (its three passes), 2: Sommerfeld outer column of a stage
   int mode;
   StageArgs a;
   FillArgs f;
@@ -80,6 +80,7 @@ struct kg_handle {
   mutable void* reduce_buf = nullptr;  // reductions of the diagnostics, kept between calls
   mutable size_t reduce_bytes = 0;
   hipEvent_t batch_done = nullptr;
+
   std::vector<RecOp>* rec = nullptr;  // when set, stage() and fill() record instead of launching
 };
 
@@ -177,6 +178,12 @@ struct Roles {
   const State5* r[R_NUM] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
+#define KG_TRY(expr)              \
+  do {                            \
+    kg_status s_ = (expr);        \
+    if (s_ != KG_OK) return s_;   \
+  } while (0)
+
 kg_status stage(kg_handle* h, int mode, const Box& b, const Roles& roles) {
   StageArgs a{};
   a.G = h->G;
@@ -201,8 +208,10 @@ kg_status stage(kg_handle* h, int mode, const Box& b, const Roles& roles) {
   a.kc = h->tune_kc;
   a.tj = h->tune_tj;
   a.nsys = h->nsys;
+  const bool somm = h->G.bc_somm && somm_stage(mode);
   if (h->rec) {
     h->rec->push_back(RecOp{0, mode, a, FillArgs{}});
+    if (somm) h->rec->push_back(RecOp{2, mode, a, FillArgs{}});
     return KG_OK;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -213,6 +222,9 @@ kg_status stage(kg_handle* h, int mode, const Box& b, const Roles& roles) {
     KG_HIP(h, hipEventRecord(e0, h->stream));
   }
   KG_HIP(h, launch_stage(mode, a, h->stream));
+  // the Sommerfeld outer column after the stage kernel (it overwrites what a
+  // full-box PIRK stage computed there)
+  if (somm) KG_HIP(h, launch_somm(mode, a, h->stream));
   if (h->prof) {
     KG_HIP(h, hipEventRecord(e1, h->stream));
     h->ev_marks.push_back({0, h->ev_used - 2});
@@ -258,11 +270,6 @@ kg_status fill(kg_handle* h, const State5& u, const State5& u_p) {
   return KG_OK;
 }
 
-#define KG_TRY(expr)              \
-  do {                            \
-    kg_status s_ = (expr);        \
-    if (s_ != KG_OK) return s_;   \
-  } while (0)
 
 // rk4 (lines_method_lib.f90:424-446) with the increments k_s = rhs*dt kept
 // instead of the stage states: stage s reads u_p + c*k_{s-1} (and its ghost
@@ -485,6 +492,8 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
     if (h->desc.scheme != scheme) return fail(lead, KG_ERR_UNSUPPORTED, "batch: handle %d runs another scheme", q);
     if (h->G.fd4 != lead->G.fd4) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed operator sets");
     if (h->G.metric_ef != ef) return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed metric forms");
+    if (h->G.bc_somm != lead->G.bc_somm)
+      return fail(lead, KG_ERR_UNSUPPORTED, "batch: mixed outer boundaries (Sommerfeld and filled)");
     if (h->device != lead->device) return fail(lead, KG_ERR_INVALID, "batch: handles on different devices");
     for (int p = 0; p < q; ++p)
       if (hs[p] == h) return fail(lead, KG_ERR_INVALID, "batch: handle %d repeated", q);
@@ -541,6 +550,11 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
           finalize_tiles(a);
           sd[vo * n + q] = a;
           tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + a.nti * a.ntj * a.ntk * a.nsys;
+        } else if (op.kind == 2) {
+          // Sommerfeld column: blocks of somm_block() work items
+          sd[vo * n + q] = op.a;
+          const int64_t nb = (somm_items(op.a) + somm_block() - 1) / somm_block();
+          tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + (int)nb;
         } else {
           fd[vo * n + q] = op.f;
         }
@@ -610,6 +624,9 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
       if (kind == 0) {
         KG_HIP(lead, launch_stage_batch(rec[0][o].mode, dsd + vo * n, dts + vo * (n + 1), n, total[vo],
                                         stage_tj(rec[0][o].mode, lead->tune_tj), ef, lead->G.fd4, lead->stream));
+      } else if (kind == 2) {
+        KG_HIP(lead, launch_somm_batch(rec[0][o].mode, dsd + vo * n, dts + vo * (n + 1), n, total[vo], lead->G.fd4,
+                                       lead->stream));
       } else {
         for (int ph = 0; ph < 3; ++ph)
           KG_HIP(lead, launch_fill_batch(dfd + vo * n, dbs + ph * (n + 1), n, dni + ph * n,
@@ -617,7 +634,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
       }
       if (prof) {
         KG_HIP(lead, hipEventRecord(e1, lead->stream));
-        lead->ev_marks.push_back({kind, lead->ev_used - 2});
+        lead->ev_marks.push_back({kind == 1 ? 1 : 0, lead->ev_used - 2});  // Sommerfeld columns count as stage time
       }
     }
   }
@@ -655,7 +672,7 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
   if (d->metric != KG_METRIC_KERR_SCHILD && d->metric != KG_METRIC_SCHW_EF)
     return fail(nullptr, KG_ERR_UNSUPPORTED, "Metric %d not implemented yet (rhs_pi supports Kerr_Schild, Schw_EF)",
                 d->metric);
-  if (d->bc_rmax != KG_BC_OUTFLOW && d->bc_rmax != KG_BC_INFLOW)
+  if (d->bc_rmax != KG_BC_OUTFLOW && d->bc_rmax != KG_BC_INFLOW && d->bc_rmax != KG_BC_SOMMERFELD)
     return fail(nullptr, KG_ERR_UNSUPPORTED, "This boundary condition type for r_max is not implemented yet");
   if (!(d->xmax > d->xmin)) return fail(nullptr, KG_ERR_INVALID, "xmax must exceed xmin");
   if (!(d->dt > 0.0) && !(d->CFL > 0.0)) return fail(nullptr, KG_ERR_INVALID, "need dt > 0 or CFL > 0");
@@ -701,6 +718,8 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
   G.dt = h->hg.dt;
   G.metric_ef = d->metric == KG_METRIC_SCHW_EF;
   G.bc_inflow = d->bc_rmax == KG_BC_INFLOW;
+  G.bc_somm = d->bc_rmax == KG_BC_SOMMERFELD;
+  G.r_out = h->hg.xmin + (double)(G.ni - 1) * h->hg.dx;  // x(Nx+g) (mesh_refinement_lib.f90:130)
 
   hipError_t e = hipGetDevice(&h->device);
   if (e == hipSuccess) e = hipStreamCreate(&h->stream);
@@ -713,6 +732,7 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
   // address of the no-allocation descriptor array (once per device)
   e = hipEventCreateWithFlags(&h->batch_done, hipEventDisableTiming);
   if (e == hipSuccess) e = resolve_scratch(h->device);
+
   if (e != hipSuccess) {
     fail(nullptr, KG_ERR_HIP, "batch set-up failed: %s", hipGetErrorString(e));
     kg_destroy(h);
@@ -751,6 +771,7 @@ kg_status kg_destroy(kg_handle* h) {
   if (h->batch_dev) (void)hipFree(h->batch_dev);
   if (h->reduce_buf) (void)hipFree(h->reduce_buf);
   if (h->batch_done) (void)hipEventDestroy(h->batch_done);
+
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return KG_OK;

@@ -19,7 +19,7 @@ struct kg_handle;
 namespace kg {
 
 constexpr int kNumFields = 5;  // phi, psi_r, psi_theta, psi_phi, Pi
-constexpr int kNumTables = 11;
+constexpr int kNumTables = 12;
 
 // table slots
 enum Table : int {
@@ -33,7 +33,9 @@ enum Table : int {
   T_ASGTT = 7,   // (alpha*sqrtg)*gamma^{theta theta}
   T_GPP = 8,     // gamma^{phi phi}
   T_AM2 = 9,     // alpha*m**2
-  T_GTT = 10     // gamma^{theta theta} (diagnostics: currents_ADM, density)
+  T_GTT = 10,    // gamma^{theta theta} (diagnostics: currents_ADM, density)
+  T_VOUT = 11    // alpha*sqrt(gamma^{rr}) - beta^r: outgoing radial characteristic speed
+                 // (the Sommerfeld outer boundary)
 };
 
 struct Geom {
@@ -49,6 +51,8 @@ struct Geom {
   int metric_ef;          // 1: Schw_EF form of rhs_pi (lines_method_lib.f90:738-743)
   int bc_inflow;          // 1: 'inflow' at r_max
   int fd4;                // 1: the 4th-order operator set
+  int bc_somm;            // 1: Sommerfeld at r_max (the outer column evolves; no xR fill)
+  double r_out;           // x of the outer column i = Nx+g (Sommerfeld's 1/r)
 };
 
 // Host-side table builder (kg_metric.cpp): fills 10 tables [nj][ld] and the
@@ -120,6 +124,16 @@ struct FillArgs {
 
 // launchers (kg_kernels.hip)
 hipError_t launch_stage(int mode, const StageArgs& a, hipStream_t s);
+// Sommerfeld outer column of a stage (G.bc_somm): every stage but PIRK 1A/2A
+// (whose outer column the stage kernel computes with its edge rows) runs it
+// after the stage kernel with the same arguments.
+bool somm_stage(int mode);
+int64_t somm_items(const StageArgs& a);
+int somm_block();  // work items per workgroup of the Sommerfeld column kernels
+hipError_t launch_somm(int mode, const StageArgs& a, hipStream_t s);
+// batched: grid q owns blocks [bstart[q], bstart[q+1]) of somm_block() items
+hipError_t launch_somm_batch(int mode, const StageArgs* descs, const int* bstart, int ngrids, int nblocks, int fd4,
+                             hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase /*0 x,1 y,2 z*/, int nsys);
 // Batched launches: one launch covers every grid of descs[0..ngrids).
 // Stage: grid q owns workgroups [tstart[q], tstart[q+1]), its tiles times its

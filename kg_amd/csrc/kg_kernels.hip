@@ -191,7 +191,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
           k = (k + dis) % (Nz - 1);
         }
         if (i == 0) return Src{OFF(1, j, k), 1};
-        if (i == ni - 1) return G.bc_inflow ? Src{OFF(ni - 1, j, k), 2} : Src{OFF(ni - 2, j, k), 0};
+        if (i == ni - 1 && !G.bc_somm) return G.bc_inflow ? Src{OFF(ni - 1, j, k), 2} : Src{OFF(ni - 2, j, k), 0};
         return Src{OFF(i, j, k), 0};
       }
     }
@@ -694,7 +694,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
           k = (k + dis) % (Nz - 1);
         }
         if (i == 0) return Src{OFF(1, j, k), 1};
-        if (i == ni - 1) return G.bc_inflow ? Src{OFF(ni - 1, j, k), 2} : Src{OFF(ni - 2, j, k), 0};
+        if (i == ni - 1 && !G.bc_somm) return G.bc_inflow ? Src{OFF(ni - 1, j, k), 2} : Src{OFF(ni - 2, j, k), 0};
         return Src{OFF(i, j, k), 0};
       }
     }
@@ -1156,7 +1156,9 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, I e) {
     const int64_t sys = (int64_t)(r / 5) * G.sys_stride;
     gdouble* u = gptr(a.u[c]) + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
     u[0] = 3.0 * u[1] - 3.0 * u[2] + u[3];
-    if (G.bc_inflow) {
+    if (G.bc_somm) {
+      // Sommerfeld: the outer column is evolved, not filled
+    } else if (G.bc_inflow) {
       gcdouble* up = gptr(a.u_p[c]) + sys + (int64_t)kk * G.plane + (int64_t)jj * G.ld;
       u[G.ni - 1] = up[G.ni - 1];
     } else {
@@ -1217,6 +1219,215 @@ __global__ __launch_bounds__(256) void fill_kernel_batch(const FillArgs* __restr
   const int q = find_grid(bstart, ngrids, (int)blockIdx.x);
   const int64_t e = (int64_t)(blockIdx.x - bstart[q]) * 256 + threadIdx.x;
   if (e < nitems[q]) fill_item<PHASE, I>(descs[q], (I)e);
+}
+
+// ---------------------------------------------------------------------------
+// Sommerfeld outer boundary (KG_BC_SOMMERFELD; the reference has 'outflow'
+// and 'inflow' only, lines_method_lib.f90:522-535).  The outer column
+// N = ni-1 is evolved instead of filled: phi and the psi fields with their own
+// right-hand sides (lines_method_lib.f90:671-713) using the operator set's
+// one-sided r rows (finite_differences.f90:36-42,779-799 / :124-151,922-957),
+// Pi with the radiative condition
+//     R_Pi = -(v * (D_r Pi + Pi / r)),   v = alpha*sqrt(gamma^rr) - beta^r,
+// D_r the same one-sided edge row, r = x(N).  One work item per (system,
+// plane, theta row) of the stage's box at column N; the stage combine is the
+// stage kernel's.  PIRK stages 1A and 2A (phi and psi over the full box) are
+// left to their stage kernel, which computes column N with the same edge rows.
+// ---------------------------------------------------------------------------
+template <int MODE, int FD4>
+__device__ __forceinline__ void somm_item(const StageArgs& a, int64_t e) {
+  constexpr bool N14 = ModeTraits<MODE>::n14;
+  constexpr bool N5 = ModeTraits<MODE>::n5;
+  constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
+  constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
+  static_assert(!N14 || Combine<MODE>::boxed, "phi and psi of the outer column are computed for boxed stages only");
+  const Geom& G = a.G;
+  const int nbj = a.j_hi - a.j_lo, nbk = a.k_hi - a.k_lo;
+  const int jj = a.j_lo + (int)(e % nbj);
+  const int64_t rest = e / nbj;
+  const int kk = a.k_lo + (int)(rest % nbk);
+  const int sysidx = (int)(rest / nbk);
+  const int ni = G.ni, nj = G.nj, Nz = G.Nz, Ny = G.Ny, g = G.g, ld = G.ld;
+  const int N = ni - 1;
+  const uint32_t plane8 = (uint32_t)(G.plane * 8);
+  const uint32_t n2b = (uint32_t)((int64_t)nj * ld * 8);
+  const uint32_t sysb = (uint32_t)((int64_t)sysidx * G.sys_stride * 8);
+  const int dis = Nz / 2;
+  auto T = [&](int which, int i, int j) { return gld(a.tab, (uint32_t)((j * ld + i) * 8) + (uint32_t)which * n2b); };
+  auto FP = [&](int r, int f) { return a.rb[r] + (int64_t)(f == 4 ? a.ri4[r] : f) * a.per; };
+  auto OFF = [&](int i, int j, int k) { return (uint32_t)((j * ld + i) * 8) + (uint32_t)k * plane8; };
+  auto E = [&](int f, uint32_t o) {
+    if constexpr (VIRT) {
+      const double p = gld(FP(R_IN, f), o + sysb), k = gld(FP(R_KIN, f), o + sysb);
+      if constexpr (MODE == M_RK4_S4)
+        return p + k;
+      else
+        return p + 0.50 * k;
+    } else {
+      return gld(FP(R_IN, f), o + sysb);
+    }
+  };
+  // stage input at (i, j, k) with the ghost fill resolved as the stage kernels
+  // resolve it (no xR fill: the outer column holds its evolved value)
+  auto U = [&](int f, int i, int j, int k) -> double {
+    if constexpr (VIRT) {
+      if (i < 1 || j < g + 1 || j > g + Ny - 1 || k < 1 || k > Nz - 1) {
+        if (k == 0)
+          k = Nz - 1;
+        else if (k == Nz)
+          k = 1;
+        if (j <= g) {
+          j = 2 * g + 1 - j;
+          k = (k + dis) % (Nz - 1);
+        } else if (j >= g + Ny) {
+          j = 2 * Ny + 2 * g - 1 - j;
+          k = (k + dis) % (Nz - 1);
+        }
+        if (i == 0) {
+          const uint32_t o = OFF(1, j, k);
+          return 3.0 * E(f, o) - 3.0 * E(f, o + 8) + E(f, o + 16);
+        }
+      }
+    }
+    return E(f, OFF(i, j, k));
+  };
+  const double rr = G.rr, rt = G.rt, rp = G.rp, dt = G.dt;
+  const double r4r = G.r4r, r4t = G.r4t, r4p = G.r4p, rar = G.rar;
+  double R[5] = {0, 0, 0, 0, 0};
+  double cur[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int f = CLO; f < CHI; ++f) cur[f] = U(f, N, jj, kk);
+  if constexpr (N14) {
+    const double al = T(T_ALPHA, N, jj), be = T(T_BETA, N, jj), dxb = T(T_DXBETA, N, jj);
+    auto F = [&](int j, int k) { return T(T_ALPHA, N, j) * U(4, N, j, k) + T(T_BETA, N, j) * U(1, N, j, k); };
+    double A[5], P[5];
+    constexpr int M = FD4 ? 5 : 3;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      A[m] = T(T_ALPHA, N - m, jj) * U(4, N - m, jj, kk);
+      P[m] = U(1, N - m, jj, kk);
+    }
+    double dr, ar;
+    if constexpr (FD4) {
+      dr = (+25.0 * A[0] - 48.0 * A[1] + 36.0 * A[2] - 16.0 * A[3] + 3.0 * A[4]) * r4r;
+      ar = (25.0 * P[0] - 48.0 * P[1] + 36.0 * P[2] - 16.0 * P[3] + 3.0 * P[4]) * rar / 12.0;
+    } else {
+      dr = (+3.0 * A[0] - 4.0 * A[1] + A[2]) * rr;
+      ar = (3.0 * P[0] - 4.0 * P[1] + P[2]) * rr;
+    }
+    R[0] = al * cur[4] + be * cur[1];
+    R[1] = dr + be * ar + dxb * cur[1];
+    if constexpr (FD4) {
+      R[2] = (F(jj - 2, kk) - 8.0 * F(jj - 1, kk) + 8.0 * F(jj + 1, kk) - F(jj + 2, kk)) * r4t;
+      if (kk == 0)
+        R[3] = (-25.0 * F(jj, 0) + 48.0 * F(jj, 1) - 36.0 * F(jj, 2) + 16.0 * F(jj, 3) - 3.0 * F(jj, 4)) * r4p;
+      else if (kk == 1)
+        R[3] = (-3.0 * F(jj, 0) - 10.0 * F(jj, 1) + 18.0 * F(jj, 2) - 6.0 * F(jj, 3) + 1.0 * F(jj, 4)) * r4p;
+      else if (kk == Nz - 1)
+        R[3] = (+3.0 * F(jj, Nz) + 10.0 * F(jj, Nz - 1) - 18.0 * F(jj, Nz - 2) + 6.0 * F(jj, Nz - 3) -
+                1.0 * F(jj, Nz - 4)) * r4p;
+      else
+        R[3] = (F(jj, kk - 2) - 8.0 * F(jj, kk - 1) + 8.0 * F(jj, kk + 1) - F(jj, kk + 2)) * r4p;
+    } else {
+      R[2] = (-F(jj - 1, kk) + F(jj + 1, kk)) * rt;
+      R[3] = (-F(jj, kk == 0 ? Nz - 2 : kk - 1) + F(jj, kk + 1)) * rp;
+    }
+  }
+  if constexpr (N5) {
+    double P[5];
+    constexpr int M = FD4 ? 5 : 3;
+#pragma unroll
+    for (int m = 0; m < M; ++m) P[m] = U(4, N - m, jj, kk);
+    double d;
+    if constexpr (FD4)
+      d = (+25.0 * P[0] - 48.0 * P[1] + 36.0 * P[2] - 16.0 * P[3] + 3.0 * P[4]) * r4r;
+    else
+      d = (+3.0 * P[0] - 4.0 * P[1] + P[2]) * rr;
+    const double v = T(T_VOUT, N, jj);
+    R[4] = -(v * (d + P[0] / G.r_out));
+  }
+  const uint32_t so = OFF(N, jj, kk) + sysb;
+  double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+  if constexpr (CBASE) {
+#pragma unroll
+    for (int f = CLO; f < CHI; ++f) vb[f] = gld(FP(R_BASE, f), so);
+  }
+  if constexpr (CAUX) {
+#pragma unroll
+    for (int f = CLO; f < CHI; ++f) va[f] = gld(FP(R_AUX, f), so);
+  }
+  if constexpr (MODE == M_RK4_S4) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      va2[f] = gld(FP(R_AUX2, f), so);
+      vk[f] = gld(FP(R_KIN, f), so);
+    }
+  }
+  if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, R[f] * dt);
+  } else if constexpr (MODE == M_RK4_S4) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * va2[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
+  } else if constexpr (MODE == M_RK3_S1) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + R[f] * dt);
+  } else if constexpr (MODE == M_RK3_S2) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, 0.750 * vb[f] + 0.250 * (cur[f] + R[f] * dt));
+  } else if constexpr (MODE == M_RK3_S3) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] / 3.0 + 2.0 * (cur[f] + R[f] * dt) / 3.0);
+  } else if constexpr (MODE == M_PIRK_1B) {
+    gst(FP(R_OUT, 4), so, vb[4] + dt * R[4]);
+  } else if constexpr (MODE == M_PIRK2_2) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, 0.5 * (vb[f] + cur[f] + dt * R[f]));
+  } else if constexpr (MODE == M_PIRK3_2B) {
+    gst(FP(R_OUT, 4), so, 0.75 * vb[4] + 0.25 * (va[4] + dt * R[4]));
+  } else if constexpr (MODE == M_PIRK3_3) {
+    const double third = 1.0 / 3.0, two_thirds = 2.0 / 3.0;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, third * vb[f] + two_thirds * (cur[f] + dt * R[f]));
+  }
+}
+
+// one wavefront per workgroup: a column has few items (nj x nk per system),
+// and many small workgroups spread them over the CUs
+constexpr int SOMM_BLOCK = 64;
+
+__host__ __device__ inline int64_t somm_count(const StageArgs& a) {
+  return (int64_t)a.nsys * (a.j_hi - a.j_lo) * (a.k_hi - a.k_lo);
+}
+
+template <int MODE, int FD4>
+__global__ __launch_bounds__(SOMM_BLOCK) void somm_kernel(const StageArgs a, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * SOMM_BLOCK + threadIdx.x;
+  if (e < n) somm_item<MODE, FD4>(a, e);
+}
+
+template <int MODE, int FD4>
+__global__ __launch_bounds__(SOMM_BLOCK) void somm_kernel_batch(const StageArgs* __restrict__ descs,
+                                                         const int* __restrict__ bstart, int ngrids) {
+  const int q = find_grid(bstart, ngrids, (int)blockIdx.x);
+  const StageArgs& a = descs[q];
+  const int64_t e = (int64_t)(blockIdx.x - bstart[q]) * SOMM_BLOCK + threadIdx.x;
+  if (e < somm_count(a)) somm_item<MODE, FD4>(a, e);
+}
+
+template <int MODE, int FD4>
+hipError_t launch_somm_mode(const StageArgs& a, hipStream_t s) {
+  const int64_t n = somm_count(a);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((somm_kernel<MODE, FD4>), dim3((unsigned)((n + SOMM_BLOCK - 1) / SOMM_BLOCK)), dim3(SOMM_BLOCK), 0, s, a, n);
+  return hipGetLastError();
+}
+
+template <int MODE, int FD4>
+hipError_t launch_somm_batch_mode(const StageArgs* descs, const int* bstart, int ngrids, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((somm_kernel_batch<MODE, FD4>), dim3((unsigned)nblocks), dim3(SOMM_BLOCK), 0, s, descs, bstart, ngrids);
+  return hipGetLastError();
 }
 
 // dense [sys][5][nk][nj][ni] <-> pitched fields
@@ -1430,5 +1641,67 @@ hipError_t launch_pack(const Geom& G, double* src, double* const d[5], int nsys,
                      0, s, G, src, d[0], d[1], d[2], d[3], d[4], to_dev, fortran, nsys);
   return hipGetLastError();
 }
+
+}  // namespace kg
+
+namespace kg {
+
+bool somm_stage(int mode) { return mode != M_PIRK_1A && mode != M_PIRK3_2A; }
+
+int64_t somm_items(const StageArgs& a) { return somm_count(a); }
+
+int somm_block() { return SOMM_BLOCK; }
+
+#define KG_SOMM_CASES(CALL)                                  \
+  switch (mode) {                                            \
+    case M_RK4_S1: return CALL(M_RK4_S1);                    \
+    case M_RK4_S2: return CALL(M_RK4_S2);                    \
+    case M_RK4_S3: return CALL(M_RK4_S3);                    \
+    case M_RK4_S4: return CALL(M_RK4_S4);                    \
+    case M_RK3_S1: return CALL(M_RK3_S1);                    \
+    case M_RK3_S2: return CALL(M_RK3_S2);                    \
+    case M_RK3_S3: return CALL(M_RK3_S3);                    \
+    default: break;                                          \
+  }
+
+hipError_t launch_somm(int mode, const StageArgs& a, hipStream_t s) {
+  if (a.G.fd4) {
+#define KG_SOMM_FD4(M) launch_somm_mode<M, 1>(a, s)
+    KG_SOMM_CASES(KG_SOMM_FD4)
+#undef KG_SOMM_FD4
+    return hipErrorInvalidValue;
+  }
+#define KG_SOMM_REF2(M) launch_somm_mode<M, 0>(a, s)
+  KG_SOMM_CASES(KG_SOMM_REF2)
+#undef KG_SOMM_REF2
+  switch (mode) {
+    case M_PIRK_1B: return launch_somm_mode<M_PIRK_1B, 0>(a, s);
+    case M_PIRK2_2: return launch_somm_mode<M_PIRK2_2, 0>(a, s);
+    case M_PIRK3_2B: return launch_somm_mode<M_PIRK3_2B, 0>(a, s);
+    case M_PIRK3_3: return launch_somm_mode<M_PIRK3_3, 0>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_somm_batch(int mode, const StageArgs* descs, const int* bstart, int ngrids, int nblocks, int fd4,
+                             hipStream_t s) {
+  if (fd4) {
+#define KG_SOMM_FD4(M) launch_somm_batch_mode<M, 1>(descs, bstart, ngrids, nblocks, s)
+    KG_SOMM_CASES(KG_SOMM_FD4)
+#undef KG_SOMM_FD4
+    return hipErrorInvalidValue;
+  }
+#define KG_SOMM_REF2(M) launch_somm_batch_mode<M, 0>(descs, bstart, ngrids, nblocks, s)
+  KG_SOMM_CASES(KG_SOMM_REF2)
+#undef KG_SOMM_REF2
+  switch (mode) {
+    case M_PIRK_1B: return launch_somm_batch_mode<M_PIRK_1B, 0>(descs, bstart, ngrids, nblocks, s);
+    case M_PIRK2_2: return launch_somm_batch_mode<M_PIRK2_2, 0>(descs, bstart, ngrids, nblocks, s);
+    case M_PIRK3_2B: return launch_somm_batch_mode<M_PIRK3_2B, 0>(descs, bstart, ngrids, nblocks, s);
+    case M_PIRK3_3: return launch_somm_batch_mode<M_PIRK3_3, 0>(descs, bstart, ngrids, nblocks, s);
+  }
+  return hipErrorInvalidValue;
+}
+#undef KG_SOMM_CASES
 
 }  // namespace kg

@@ -168,6 +168,7 @@ int build_tables(const HostGrid& G, int metric, double a, double M_bh, double m,
       tab[T_GPP * n2 + p] = gpp;
       tab[T_AM2 * n2 + p] = alpha * (m * m);
       tab[T_GTT * n2 + p] = gtt;
+      tab[T_VOUT * n2 + p] = alpha * std::sqrt(grr) - beta[1];
       beta_row[ii] = beta[1];
     }
     // dx_beta = Re dx1(beta^r) (geometry.f90:344; stencil finite_differences.f90:36-42);

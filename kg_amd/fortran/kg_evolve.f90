@@ -6,8 +6,9 @@
 !   in.bin/out.bin: u(5, -g:Nx+g, -g:Ny+g, 0:Nz) complex(8), raw stream
 !   in.bin may instead be ic=<type>,<mu>,<sigma>,<ncart>,<seed>: the initial
 !   data of main/initial_conditions.f90 built by the engine
-!   optional 11th argument batch=<K>: K copies of the grid advanced together in
-!   batched launches; the driver stops with an error unless all K agree
+!   optional further arguments: batch=<K>: K copies of the grid advanced
+!   together in batched launches (the driver stops with an error unless all K
+!   agree); bc=<outflow|inflow|sommerfeld>: the outer boundary (outflow)
 program kg_evolve
   use kg_hip_mod
   implicit none
@@ -15,6 +16,8 @@ program kg_evolve
   type(kg_lines_method), allocatable :: batch(:)
   complex(kind=8), allocatable :: u(:,:,:,:), v(:,:,:,:)
   character(len=256) :: arg, fin, fout, scheme
+  character(len=16) :: bc
+  integer :: ia
   integer :: Nx, Ny, Nz, nsteps, l, g, isch, ncart, ops, nb, q
   integer(kind=8) :: seed
   real(kind=8) :: xmin, xmax, m, mu, sigma, amp
@@ -32,10 +35,12 @@ program kg_evolve
   call get_command_argument(9, fin)
   call get_command_argument(10, fout)
   nb = 1
-  if (command_argument_count() >= 11) then
-    call get_command_argument(11, arg)
+  bc = 'outflow'
+  do ia = 11, command_argument_count()
+    call get_command_argument(ia, arg)
     if (arg(1:6) == 'batch=') read(arg(7:), *) nb
-  end if
+    if (arg(1:3) == 'bc=') bc = trim(arg(4:))
+  end do
 
   ops = KG_OPS_REF2
   select case (trim(scheme))
@@ -48,7 +53,7 @@ program kg_evolve
   end select
 
   MoL = kg_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
-                                    'outflow', 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., ops)
+                                    trim(bc), 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., ops)
   allocate(u(5, -g:Nx+g, -g:Ny+g, 0:Nz))
   if (fin(1:3) == 'ic=') then
     read(fin(4:), *) ictype, mu, sigma, ncart, seed
@@ -66,7 +71,7 @@ program kg_evolve
     batch(1) = MoL
     do q = 2, nb
       batch(q) = kg_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
-                                             'outflow', 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., ops)
+                                             trim(bc), 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., ops)
       if (fin(1:3) == 'ic=') then
         call batch(q)%initial_conditions(trim(ictype), mu, sigma, ncart, seed, amp)
       else

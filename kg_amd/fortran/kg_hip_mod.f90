@@ -20,7 +20,7 @@ module kg_hip_mod
   integer(c_int32_t), parameter, public :: KG_SCHEME_PIRK2 = 0, KG_SCHEME_PIRK3 = 1, KG_SCHEME_RK3 = 2, KG_SCHEME_RK4 = 3
   integer(c_int32_t), parameter, public :: KG_METRIC_MINKOWSKI = 0, KG_METRIC_KERR_SCHILD = 1, &
                                            KG_METRIC_MKS = 2, KG_METRIC_SCHW_EF = 3
-  integer(c_int32_t), parameter, public :: KG_BC_OUTFLOW = 0, KG_BC_INFLOW = 1
+  integer(c_int32_t), parameter, public :: KG_BC_OUTFLOW = 0, KG_BC_INFLOW = 1, KG_BC_SOMMERFELD = 2
   integer(c_int32_t), parameter, public :: KG_LAYOUT_FORTRAN = 0, KG_LAYOUT_SOA = 1
   integer(c_int32_t), parameter, public :: KG_OPS_REF2 = 0, KG_OPS_FD4 = 1
 
@@ -224,6 +224,8 @@ contains
       d%bc_rmax = KG_BC_OUTFLOW
     else if (boundary_type_rmax == 'inflow') then
       d%bc_rmax = KG_BC_INFLOW
+    else if (boundary_type_rmax == 'sommerfeld') then
+      d%bc_rmax = KG_BC_SOMMERFELD     ! radiative outer boundary (not in the reference)
     else
       print*, 'This boundary condition type for r_max is not implemented yet'
       print*, 'try with: outflow or inflow'

@@ -21,7 +21,9 @@ from . import _capi as capi
 _SCHEMES = {"pirk2": capi.KG_PIRK2, "pirk3": capi.KG_PIRK3, "rk3": capi.KG_RK3, "rk4": capi.KG_RK4}
 _METRICS = {"Minkowski": capi.KG_METRIC_MINKOWSKI, "Kerr_Schild": capi.KG_METRIC_KERR_SCHILD,
             "MKS": capi.KG_METRIC_MKS, "Schw_EF": capi.KG_METRIC_SCHW_EF}
-_BCS = {"outflow": capi.KG_BC_OUTFLOW, "inflow": capi.KG_BC_INFLOW}
+# 'sommerfeld': a radiative outer boundary the reference does not have (the
+# outer column is evolved with d_t Pi = -v (d_r Pi + Pi/r)); see include/kg.h
+_BCS = {"outflow": capi.KG_BC_OUTFLOW, "inflow": capi.KG_BC_INFLOW, "sommerfeld": capi.KG_BC_SOMMERFELD}
 # operator set of the RHS: the reference's live 2nd-order stencils, or the
 # 4th-order set its finite_differences.f90 defines (RK3 and RK4 only)
 _OPS = {"ref2": capi.KG_OPS_REF2, "fd4": capi.KG_OPS_FD4}

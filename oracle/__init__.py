@@ -24,7 +24,8 @@ PIRK2, PIRK3, RK3, RK4 = 0, 1, 2, 3
 SCHEMES = {"pirk2": PIRK2, "pirk3": PIRK3, "rk3": RK3, "rk4": RK4}
 MINKOWSKI, KERR_SCHILD, MKS, SCHW_EF = 0, 1, 2, 3
 METRICS = {"Minkowski": MINKOWSKI, "Kerr_Schild": KERR_SCHILD, "MKS": MKS, "Schw_EF": SCHW_EF}
-OUTFLOW, INFLOW = 0, 1
+OUTFLOW, INFLOW, SOMMERFELD = 0, 1, 2
+BCS = {"outflow": OUTFLOW, "inflow": INFLOW, "sommerfeld": SOMMERFELD}
 # deliberate mistakes the oracle can plant (kg_oracle.h KGO_PLANT_*), for the
 # tests' mutation checks
 PLANTS = {"drop_pi_dxbeta": 1, "swap_phi_flux": 2, "psi2_alpha_pi": 3, "drop_psi1_dxbeta": 4,
@@ -105,8 +106,7 @@ class Sim:
     def __init__(self, Nx, Ny, Nz, g=2, xmin=1.8, xmax=60.0, metric="Kerr_Schild", a=0.0, M_bh=1.0,
                  m=0.0, bc="outflow", CFL=0.25, dt=0.0):
         L = lib()
-        self._h = L.kgo_create(Nx, Ny, Nz, g, xmin, xmax, METRICS[metric], a, M_bh, m,
-                               OUTFLOW if bc == "outflow" else INFLOW, CFL, dt)
+        self._h = L.kgo_create(Nx, Ny, Nz, g, xmin, xmax, METRICS[metric], a, M_bh, m, BCS[bc], CFL, dt)
         if not self._h:
             raise ValueError("invalid oracle configuration")
         o = (C.c_int64 * 8)()

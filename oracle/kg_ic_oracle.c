@@ -206,7 +206,8 @@ static void fill_phase(kgo_sim* s, double* u, int phase) {
           a[0] = 3.0 * a[1] - 3.0 * a[2] + a[3];
           /* inflow copies u_p, which the reference has not written yet at
            * this point: taken as zero */
-          a[ni - 1] = s->bc_rmax == KGO_BC_OUTFLOW ? a[ni - 2] : 0.0;
+          if (s->bc_rmax != KGO_BC_SOMMERFELD)  /* Sommerfeld keeps the outer column's own value */
+            a[ni - 1] = s->bc_rmax == KGO_BC_OUTFLOW ? a[ni - 2] : 0.0;
         }
     } else if (phase == 1) {
       for (int k = 0; k <= Nz; ++k) {

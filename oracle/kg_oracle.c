@@ -645,6 +645,23 @@ void kgo_rhs_pi(kgo_sim* s, double* rhs, const double* vars) {
         const double mass = (s->plant == KGO_PLANT_MASS_SIGN) ? -(T2(s->alpha) * m2 * v1[p]) : T2(s->alpha) * m2 * v1[p];
         rhs[p] = d1[p] / sg + d2[p] / sg + t3 + pdb + T2(s->beta_r) * ad[p] / sg - mass;
       }
+  if (s->bc_rmax == KGO_BC_SOMMERFELD) {
+    /* the radiative condition at the outer column (no reference counterpart) */
+    const int N = s->ni - 1;
+    const double r = s->x[N];
+    for (int kk = 0; kk < s->nk; ++kk)
+      for (int jj = 0; jj < s->nj; ++jj) {
+        const int64_t p = IDX(s, N, jj, kk), p2 = IDX2(s, N, jj);
+        const double* a = v5 + p;  /* a[-m] = Pi at column N-m */
+        double d;
+        if (s->order == 4)
+          d = (+25.0 * a[0] - 48.0 * a[-1] + 36.0 * a[-2] - 16.0 * a[-3] + 3.0 * a[-4]) * (1.0 / (12.0 * s->dx));
+        else
+          d = (+3.0 * a[0] - 4.0 * a[-1] + a[-2]) * (0.50 / s->dx);
+        const double v = s->alpha[p2] * sqrt(s->g_rr[p2]) - s->beta_r[p2];
+        rhs[p] = -(v * (d + a[0] / r));
+      }
+  }
 }
 
 void kgo_rhs_all(kgo_sim* s, double* rhs5, const double* vars) {
@@ -671,8 +688,8 @@ void kgo_fill(kgo_sim* s, double* u, const double* u_p) {
         double* a = f + IDX(s, 0, jj, kk);
         a[0] = 3.0 * a[1] - 3.0 * a[2] + a[3];
       }
-    /* boundary_xR :525-528 */
-    for (int kk = 0; kk < nk; ++kk)
+    /* boundary_xR :525-528 (none with Sommerfeld: the outer column is evolved) */
+    for (int kk = 0; kk < nk && s->bc_rmax != KGO_BC_SOMMERFELD; ++kk)
       for (int jj = 0; jj < nj; ++jj) {
         int64_t o = IDX(s, ni - 1, jj, kk);
         if (s->bc_rmax == KGO_BC_OUTFLOW)

@@ -43,7 +43,13 @@ extern "C" {
 #endif
 
 enum { KGO_METRIC_MINKOWSKI = 0, KGO_METRIC_KERR_SCHILD = 1, KGO_METRIC_MKS = 2, KGO_METRIC_SCHW_EF = 3 };
-enum { KGO_BC_OUTFLOW = 0, KGO_BC_INFLOW = 1 };
+/* KGO_BC_SOMMERFELD has no counterpart in the reference (which has 'outflow'
+ * and 'inflow' only, lines_method_lib.f90:522-535): the outer column
+ * ii = ni-1 is not filled, and R_Pi there is the radiative condition
+ * -(v * (D_r Pi + Pi / r)), v = alpha*sqrt(gamma^rr) - beta^r, D_r the
+ * operator set's last-row one-sided stencil, r = x(ni-1); the other fields
+ * keep their right-hand sides (edge rows) there. */
+enum { KGO_BC_OUTFLOW = 0, KGO_BC_INFLOW = 1, KGO_BC_SOMMERFELD = 2 };
 enum { KGO_PIRK2 = 0, KGO_PIRK3 = 1, KGO_RK3 = 2, KGO_RK4 = 3 };
 
 typedef struct kgo_sim {

@@ -167,16 +167,18 @@ def test_fd4_rejections():
 
 def test_fd4_small_reference_workload_2000_steps():
     """The small workload as specified (Nr=256, Ntheta=32, Nphi=64, r in [1.8, 60],
-    massless, Kerr-Schild M=1, RK4 at CFL 0.25, 4th-order stencils, 2000 steps,
-    phi = 1e-3 exp(-(r-15)^2/1.5^2) Y_2^0, Pi = 0): bit-identical to the oracle,
-    so the per-field relative error is 0 against the 1e-12 bar."""
+    massless, Kerr-Schild M=1, RK4 at CFL 0.25, 4th-order stencils, Sommerfeld
+    outer boundary, 2000 steps, phi = 1e-3 exp(-(r-15)^2/1.5^2) Y_2^0, Pi = 0):
+    bit-identical to the oracle, so the per-field relative error is 0 against
+    the 1e-12 bar."""
     from kg_amd.initial_data import small_workload_state
-    print("fd4 small workload, 2000 steps", flush=True)
-    sim = oracle.Sim(256, 33, 65, 2, 1.8, 60.0, "Kerr_Schild", 0.0, 1.0, 0.0, "outflow", 0.25)
+    print("fd4 small workload, Sommerfeld, 2000 steps", flush=True)
+    sim = oracle.Sim(256, 33, 65, 2, 1.8, 60.0, "Kerr_Schild", 0.0, 1.0, 0.0, "sommerfeld", 0.25)
     sim.set_order(4)
     u0 = sim.fill(small_workload_state())
     sim.u[...] = u0
-    gpu = LinesMethod(Nx=256, Ny=33, Nz=65, xmin=1.8, xmax=60.0, CFL=0.25, scheme="rk4", operators="fd4")
+    gpu = LinesMethod(Nx=256, Ny=33, Nz=65, xmin=1.8, xmax=60.0, CFL=0.25, scheme="rk4", operators="fd4",
+                      boundary_type_rmax="sommerfeld")
     gpu.set_u_soa(u0[None])
     for _ in range(4):
         sim.step("rk4", 500)

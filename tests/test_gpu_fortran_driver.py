@@ -19,10 +19,13 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built (no amdflang)")]
 
 
-@pytest.mark.parametrize("scheme,batch", [("pirk2", 1), ("rk4", 1), ("rk4_fd4", 1), ("pirk2", 3), ("rk4_fd4", 2)])
-def test_fortran_driver_matches_oracle(tmp_path, scheme, batch):
+@pytest.mark.parametrize("scheme,batch,bc", [("pirk2", 1, "outflow"), ("rk4", 1, "outflow"), ("rk4_fd4", 1, "outflow"),
+                                             ("pirk2", 3, "outflow"), ("rk4_fd4", 2, "outflow"),
+                                             ("pirk3", 1, "sommerfeld"), ("rk4_fd4", 2, "sommerfeld")])
+def test_fortran_driver_matches_oracle(tmp_path, scheme, batch, bc):
     """batch > 1: the driver advances that many copies through
-    kg_lines_method_batch_step (kg_batch_step) and checks they agree."""
+    kg_lines_method_batch_step (kg_batch_step) and checks they agree; bc: the
+    outer boundary the driver passes to the constructor."""
     Nx, Ny, Nz, xmin, xmax, m = 20, 9, 17, 1.8, 10.0, 0.2
     re = shell_state(Nx, Ny, Nz, 2, xmin, xmax, 5.0, 1.2, 1.0, {(2, 0): 1.0, (1, -1): 0.4})
     im = shell_state(Nx, Ny, Nz, 2, xmin, xmax, 6.0, 1.0, 0.3, {(1, 1): 1.0})
@@ -32,7 +35,7 @@ def test_fortran_driver_matches_oracle(tmp_path, scheme, batch):
     uf.ravel(order="F").tofile(fin)
     nsteps = 6
     print("fortran driver", scheme, "batch", batch, flush=True)
-    extra = ["batch=%d" % batch] if batch > 1 else []
+    extra = (["batch=%d" % batch] if batch > 1 else []) + (["bc=" + bc] if bc != "outflow" else [])
     r = subprocess.run([EXE, str(Nx), str(Ny), str(Nz), repr(xmin), repr(xmax), repr(m), scheme, str(nsteps),
                         str(fin), str(fout)] + extra, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -41,7 +44,7 @@ def test_fortran_driver_matches_oracle(tmp_path, scheme, batch):
     got = np.fromfile(fout, dtype=np.complex128).reshape(uf.shape, order="F")
     got = np.transpose(got, (0, 3, 2, 1))
     for part, arr in (("re", re), ("im", im)):
-        sim = oracle.Sim(Nx, Ny, Nz, 2, xmin, xmax, "Kerr_Schild", 0.0, 1.0, m, "outflow", 0.25)
+        sim = oracle.Sim(Nx, Ny, Nz, 2, xmin, xmax, "Kerr_Schild", 0.0, 1.0, m, bc, 0.25)
         if scheme.endswith("_fd4"):
             sim.set_order(4)
         sim.u[...] = arr
