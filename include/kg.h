@@ -324,6 +324,36 @@ kg_status kg_ic_random_field(int32_t device, int32_t ncart, double r_max, double
 const char* kg_ic_last_error(void);
 
 /* ------------------------------------------------------------------------
+ * Two fixed refinement levels (no reference counterpart: the reference's
+ * mesh_refinement_lib.f90:16-147 is a slab decomposition of one uniform
+ * grid).  A base grid (`base`, any scheme and operator set) and a 2:1
+ * radially refined level on [xmin, r_patch] with the same angular grid, dr/2
+ * and dt/2, advanced Berger-Oliger style: per base step the base level steps,
+ * then the fine level takes two steps, each after its last `buffer` columns
+ * (beyond r_patch) are refilled from the base level -- in r by 6-point
+ * interpolation (3, -25, 150, 150, -25, 3)/256 between base points and
+ * copies at coincident ones, in time through the last three base times
+ * (weights -1/8, 3/4, 3/8 at the half step; 1/2, 1/2 while only two are
+ * known) -- with its outer column held at its start-of-step value; then the
+ * base columns 0 .. I-3 (I: the base column at r_patch) take the fine values
+ * at the same radii.  The caller sets both levels' initial data through the
+ * handles kg_refined_levels returns (kg_upload_state, kg_fill_ghosts, ...)
+ * and reads results the same way; the levels must not be stepped directly.
+ * ------------------------------------------------------------------------ */
+typedef struct kg_refined kg_refined;
+typedef struct kg_refine_desc {
+  double r_patch;   /* outer radius of the refined patch (rounded to a base grid point) */
+  int32_t buffer;   /* fine buffer columns beyond r_patch (even, 8..64; 0: 16) */
+  int32_t reserved;
+} kg_refine_desc;
+kg_status kg_refined_create(const kg_desc* base, const kg_refine_desc* desc, kg_refined** out);
+kg_status kg_refined_levels(kg_refined* r, kg_handle** base, kg_handle** fine);
+/* nsteps base steps (each with two fine steps) */
+kg_status kg_refined_step(kg_refined* r, int32_t nsteps);
+kg_status kg_refined_destroy(kg_refined* r);
+const char* kg_refined_last_error(const kg_refined* r);
+
+/* ------------------------------------------------------------------------
  * HBM bandwidth probe (measurement only; no reference counterpart): the
  * average duration of `reps` launches of a streaming kernel over FP64 arrays
  * of n doubles (n even) on `device`, `blocks` workgroups of 256 threads.

@@ -50,6 +50,11 @@ class KGSpecDesc(C.Structure):
 KG_IC_SPHERICAL_SHELL, KG_IC_RANDOM, KG_IC_RANDOM_SHELL = range(3)
 
 
+class KGRefineDesc(C.Structure):
+    """Mirror of struct kg_refine_desc."""
+    _fields_ = [("r_patch", C.c_double), ("buffer", C.c_int32), ("reserved", C.c_int32)]
+
+
 class KGICDesc(C.Structure):
     """Mirror of struct kg_ic_desc."""
     _fields_ = [
@@ -99,6 +104,11 @@ SIGNATURES = {
     "kg_initial_data": (C.c_int32, [_vp, C.POINTER(KGICDesc), _dp]),
     "kg_ic_random_field": (C.c_int32, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, _dp, _dp]),
     "kg_ic_last_error": (C.c_char_p, []),
+    "kg_refined_create": (C.c_int32, [C.POINTER(KGDesc), C.POINTER(KGRefineDesc), C.POINTER(_vp)]),
+    "kg_refined_levels": (C.c_int32, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "kg_refined_step": (C.c_int32, [_vp, C.c_int32]),
+    "kg_refined_destroy": (C.c_int32, [_vp]),
+    "kg_refined_last_error": (C.c_char_p, [_vp]),
     "kg_hbm_probe": (C.c_int32, [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _dp]),
     "kg_probe_last_error": (C.c_char_p, []),
 }

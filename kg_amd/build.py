@@ -11,8 +11,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkg_hip.so")
 
-SOURCES = ["kg_capi.hip", "kg_kernels.hip", "kg_spectral.hip", "kg_diag.hip", "kg_initial.hip", "kg_probe.hip",
-           "kg_metric.cpp", "kg_special.cpp"]
+SOURCES = ["kg_capi.hip", "kg_kernels.hip", "kg_spectral.hip", "kg_diag.hip", "kg_initial.hip", "kg_refine.hip",
+           "kg_probe.hip", "kg_metric.cpp", "kg_special.cpp"]
 
 # -ffp-contract=off: no fused multiply-add is formed, so every operation
 # rounds on its own as in the reference's Fortran expressions.

@@ -29,6 +29,27 @@ _BCS = {"outflow": capi.KG_BC_OUTFLOW, "inflow": capi.KG_BC_INFLOW, "sommerfeld"
 _OPS = {"ref2": capi.KG_OPS_REF2, "fd4": capi.KG_OPS_FD4}
 
 
+def make_desc(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, CFL, boundary_type_rmax, metric, a, M_bh, m, scheme, dt,
+              is_complex, operators):
+    """struct kg_desc from the constructor's arguments (lines_method_lib.f90:49-51)."""
+    if boundary_type_rmax not in _BCS:
+        raise capi.KGError(capi.KG_ERR_UNSUPPORTED, "This boundary condition type for r_max is not implemented yet")
+    if metric not in _METRICS:
+        raise capi.KGError(capi.KG_ERR_UNSUPPORTED, "That metric is not implmented yet")
+    if operators not in _OPS:
+        raise capi.KGError(capi.KG_ERR_INVALID, "operators must be 'ref2' or 'fd4'")
+    d = capi.KGDesc()
+    d.nvars, d.g_pts, d.Nx, d.Ny, d.Nz = nvars, g_pts, Nx, Ny, Nz
+    d.scheme = _SCHEMES[scheme]
+    d.metric = _METRICS[metric]
+    d.bc_rmax = _BCS[boundary_type_rmax]
+    d.is_complex = 1 if is_complex else 0
+    d.op_set = _OPS[operators]
+    d.xmin, d.xmax, d.tmin, d.CFL, d.dt = xmin, xmax, tmin, CFL, dt
+    d.a, d.M_bh, d.m = a, M_bh, m
+    return d
+
+
 class LinesMethod:
     """One grid of the Klein-Gordon method-of-lines solver, resident on the GPU.
 
@@ -39,26 +60,17 @@ class LinesMethod:
 
     def __init__(self, nvars=5, g_pts=2, xmin=1.8, xmax=60.0, Nx=256, Ny=33, Nz=65, tmin=0.0, tmax=None,
                  CFL=0.25, boundary_type_rmax="outflow", metric="Kerr_Schild", a=0.0, M_bh=1.0, m=0.0,
-                 scheme="pirk2", dt=0.0, is_complex=False, operators="ref2"):
-        if boundary_type_rmax not in _BCS:
-            raise capi.KGError(capi.KG_ERR_UNSUPPORTED,
-                               "This boundary condition type for r_max is not implemented yet")
-        if metric not in _METRICS:
-            raise capi.KGError(capi.KG_ERR_UNSUPPORTED, "That metric is not implmented yet")
+                 scheme="pirk2", dt=0.0, is_complex=False, operators="ref2", _handle=None):
         self._lib = capi.load()
-        d = capi.KGDesc()
-        d.nvars, d.g_pts, d.Nx, d.Ny, d.Nz = nvars, g_pts, Nx, Ny, Nz
-        d.scheme = _SCHEMES[scheme]
-        d.metric = _METRICS[metric]
-        d.bc_rmax = _BCS[boundary_type_rmax]
-        d.is_complex = 1 if is_complex else 0
-        if operators not in _OPS:
-            raise capi.KGError(capi.KG_ERR_INVALID, "operators must be 'ref2' or 'fd4'")
-        d.op_set = _OPS[operators]
-        d.xmin, d.xmax, d.tmin, d.CFL, d.dt = xmin, xmax, tmin, CFL, dt
-        d.a, d.M_bh, d.m = a, M_bh, m
-        h = C.c_void_p()
-        capi.check(self._lib.kg_create(C.byref(d), C.byref(h)))
+        d = make_desc(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, CFL, boundary_type_rmax, metric, a, M_bh, m,
+                      scheme, dt, is_complex, operators)
+        if _handle is None:
+            h = C.c_void_p()
+            capi.check(self._lib.kg_create(C.byref(d), C.byref(h)))
+            self._owned = True
+        else:  # a level of a refined pair: the pair owns the handle
+            h = _handle
+            self._owned = False
         self._h = h
         self.desc = d
         dims = (C.c_int64 * 8)()
@@ -79,7 +91,8 @@ class LinesMethod:
     # -- lifetime -------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.kg_destroy(self._h)
+            if getattr(self, "_owned", True):
+                self._lib.kg_destroy(self._h)
             self._h = None
 
     def __del__(self):
