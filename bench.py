@@ -84,9 +84,10 @@ TRAFFIC_PROFILES = {"ref2": "stage_kernel_rk4_nr2048_traffic.json", "fd4": "stag
 
 
 def measured_traffic(tuning, operators):
-    """HBM bytes per stage launch from the committed PMC profile of this workload
-    (FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a copy
-    of known size); None when the launch shape differs from the profiled one."""
+    """Bytes per stage launch between L2 and the memory fabric, from the
+    committed PMC profile of this workload (the TCC read and write request
+    counters by request size, exact on streaming kernels of known size); None
+    when the launch shape differs from the profiled one."""
     path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILES[operators])
     if not os.path.exists(path):
         return None
@@ -344,6 +345,7 @@ def plain(operators, steps, warmup, kc=0, tj=8, scheme=None):
     t_p1 = time.perf_counter()
     stage_ms, stage_n = lm.profile_read(0)
     fill_ms, fill_n = lm.profile_read(1)
+    col_ms, col_n = lm.profile_read(2)
     lm.profile(False)
     # the drop-in boundary hands the state over in the reference's layout
     # u(5, i, j, k): one download and one upload, timed (PCIe, host buffers)
@@ -377,6 +379,7 @@ def plain(operators, steps, warmup, kc=0, tj=8, scheme=None):
                               "kernel durations from a separate eager pass with events around each kernel; "
                               "value and ms_per_step time the graph-replay path"},
             "fill_ms_per_step": fill_ms / prof_steps,
+            "outer_column_ms_per_step": col_ms / prof_steps,
             "host_transfer": {"bytes": host_bytes, "download_s": t3 - t2, "upload_s": t4 - t3,
                               "pcie_inclusive_updates_per_s": npts * steps / (elapsed + (t3 - t2) + (t4 - t3)),
                               "note": "state handed over once each way in the reference's u(5,i,j,k) layout "
@@ -451,6 +454,7 @@ def main():
         "roofline": head["roofline"],
         "kernel_timing": head["kernel_timing"],
         "fill_ms_per_step": head["fill_ms_per_step"],
+        "outer_column_ms_per_step": head["outer_column_ms_per_step"],
         "host_transfer": head["host_transfer"],
     }
     other = "ref2" if args.operators == "fd4" else "fd4"

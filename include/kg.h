@@ -191,8 +191,9 @@ kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj);
 
 /* Kernel timing with HIP events on the handle's stream (no reference
  * counterpart; the reference only prints MPI_Wtime, save_gnuplot.f90:154).
- * kind: 0 = RHS+stage kernels, 1 = ghost-fill kernels.  While enabled,
- * kg_step launches eagerly with an event pair around every kernel. */
+ * kind: 0 = RHS+stage kernels, 1 = ghost-fill kernels, 2 = Sommerfeld
+ * outer-column kernels.  While enabled, kg_step launches eagerly with an
+ * event pair around every kernel. */
 kg_status kg_profile_enable(kg_handle* h, int32_t on);
 kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t* launches);
 

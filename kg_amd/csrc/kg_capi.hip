@@ -73,8 +73,8 @@ struct kg_handle {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<std::pair<int, size_t>> ev_marks;  // (kind, index of start event)
-  double prof_ms[2] = {0.0, 0.0};
-  int64_t prof_n[2] = {0, 0};
+  double prof_ms[3] = {0.0, 0.0, 0.0};  // stage kernels, ghost fills, Sommerfeld columns
+  int64_t prof_n[3] = {0, 0, 0};
   void* batch_dev = nullptr;  // descriptors of batched launches led by this handle
   size_t batch_cap = 0;
   mutable void* reduce_buf = nullptr;  // reductions of the diagnostics, kept between calls
@@ -222,12 +222,24 @@ kg_status stage(kg_handle* h, int mode, const Box& b, const Roles& roles) {
     KG_HIP(h, hipEventRecord(e0, h->stream));
   }
   KG_HIP(h, launch_stage(mode, a, h->stream));
-  // the Sommerfeld outer column after the stage kernel (it overwrites what a
-  // full-box PIRK stage computed there)
-  if (somm) KG_HIP(h, launch_somm(mode, a, h->stream));
   if (h->prof) {
     KG_HIP(h, hipEventRecord(e1, h->stream));
     h->ev_marks.push_back({0, h->ev_used - 2});
+  }
+  // the Sommerfeld outer column after the stage kernel (it overwrites what a
+  // full-box PIRK stage computed there); timed as kind 2
+  if (somm) {
+    if (h->prof) {
+      e0 = next_event(h);
+      e1 = next_event(h);
+      if (!e0 || !e1) return fail(h, KG_ERR_HIP, "hipEventCreate failed");
+      KG_HIP(h, hipEventRecord(e0, h->stream));
+    }
+    KG_HIP(h, launch_somm(mode, a, h->stream));
+    if (h->prof) {
+      KG_HIP(h, hipEventRecord(e1, h->stream));
+      h->ev_marks.push_back({2, h->ev_used - 2});
+    }
   }
   return KG_OK;
 }
@@ -634,7 +646,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
       }
       if (prof) {
         KG_HIP(lead, hipEventRecord(e1, lead->stream));
-        lead->ev_marks.push_back({kind == 1 ? 1 : 0, lead->ev_used - 2});  // Sommerfeld columns count as stage time
+        lead->ev_marks.push_back({kind, lead->ev_used - 2});
       }
     }
   }
@@ -993,14 +1005,16 @@ kg_status kg_profile_enable(kg_handle* h, int32_t on) {
   h->prof = on != 0;
   h->ev_used = 0;
   h->ev_marks.clear();
-  h->prof_ms[0] = h->prof_ms[1] = 0.0;
-  h->prof_n[0] = h->prof_n[1] = 0;
+  for (int k = 0; k < 3; ++k) {
+    h->prof_ms[k] = 0.0;
+    h->prof_n[k] = 0;
+  }
   return KG_OK;
 }
 
 kg_status kg_profile_read(kg_handle* h, int32_t kind, double* total_ms, int64_t* launches) {
   if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
-  if (kind < 0 || kind > 1) return fail(h, KG_ERR_INVALID, "kind must be 0 or 1");
+  if (kind < 0 || kind > 2) return fail(h, KG_ERR_INVALID, "kind must be 0, 1 or 2");
   KG_HIP(h, hipStreamSynchronize(h->stream));
   for (auto& m : h->ev_marks) {
     float ms = 0.f;
