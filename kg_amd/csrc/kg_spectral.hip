@@ -26,7 +26,8 @@
 // BF16).  Slice products t_p u_q are integers below 2^16 in units of
 // 2^{-8(p+q+2)}; the products of one diagonal p + q = d (at most 5 pairs)
 // summed over the 32 radii stay below 2^24 units, so the FP32 accumulation of
-// the matrix cores is exact.  Diagonals d <= 4 are kept (15 slice products);
+// the matrix cores (one v_mfma_f32_16x16x32_bf16 per slice pair and chunk) is
+// exact.  Diagonals d <= 4 are kept (15 slice products);
 // each chunk's FP32 sums are scaled back and added in FP64.  The dropped
 // terms are below ~6 * 2^-40 of sum_i |W||flm| per output.
 #include <hip/hip_runtime.h>
@@ -291,7 +292,8 @@ __global__ __launch_bounds__(256) void radial_f64_kernel(const int2* __restrict_
 }
 
 // ---------------------------------------------------------------- radial split BF16
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ short bf16_exact(double t, float unit) {
@@ -300,8 +302,8 @@ __device__ __forceinline__ short bf16_exact(double t, float unit) {
 }
 
 // Cut 8 values (already scaled into (-1, 1)) into kSlices BF16 slices; slice p
-// of value e sits in sl[p][e/4][e%4].
-__device__ __forceinline__ void slice8(const double* v, v4s sl[kSlices][2]) {
+// of value e is element e of sl[p].
+__device__ __forceinline__ void slice8(const double* v, v8s sl[kSlices]) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     double y = v[e];
@@ -310,14 +312,18 @@ __device__ __forceinline__ void slice8(const double* v, v4s sl[kSlices][2]) {
       y *= 256.0;
       const double t = rint(y);
       y -= t;
-      sl[p][e >> 2][e & 3] = bf16_exact(t, __uint_as_float((uint32_t)(127 - 8 * (p + 1)) << 23));
+      sl[p][e] = bf16_exact(t, __uint_as_float((uint32_t)(127 - 8 * (p + 1)) << 23));
     }
   }
 }
 
-// One wave per 16 x 16 output tile {l, n0, c0}.  16x16x16 BF16 MFMA operand
-// layout: lane L holds A[L%16][4(L/16)+e], B[4(L/16)+e][L%16], and the result
+// One wave per 16 x 16 output tile {l, n0, c0}, one chunk of 32 radii per
+// 16x16x32 BF16 MFMA (gfx950).  Operand layout: lane L holds
+// A[L%16][8(L/16)+e], B[8(L/16)+e][L%16], e = 0..7, and the result
 // D[4(L/16)+e][L%16], e = 0..3.  A = W_l rows (wave numbers), B = flm columns.
+// Every partial sum of a diagonal is an integer below 2^24 units, so the FP32
+// accumulation is exact in any order and the result does not depend on how
+// the matrix core groups the 32 products.
 __global__ __launch_bounds__(256) void radial_mfma_kernel(const int4* __restrict__ tiles, int ntiles,
                                                           const double* __restrict__ W,
                                                           const double* __restrict__ flm, double* __restrict__ ft,
@@ -338,7 +344,7 @@ __global__ __launch_bounds__(256) void radial_mfma_kernel(const int4* __restrict
     double a[8], b[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int i = i0 + (e >> 2) * 16 + 4 * q + (e & 3);
+      const int i = i0 + 8 * q + e;
       a[e] = (vn && i < Nr) ? Arow[i] : 0.0;
       b[e] = (vc && i < Nr) ? Bcol[i] : 0.0;
     }
@@ -360,7 +366,7 @@ __global__ __launch_bounds__(256) void radial_mfma_kernel(const int4* __restrict
       a[e] = ldexp(a[e], -ea);
       b[e] = ldexp(b[e], -eb);
     }
-    v4s as[kSlices][2], bs[kSlices][2];
+    v8s as[kSlices], bs[kSlices];
     slice8(a, as);
     slice8(b, bs);
     v4f acc[kSlices];
@@ -371,8 +377,8 @@ __global__ __launch_bounds__(256) void radial_mfma_kernel(const int4* __restrict
 #pragma unroll
       for (int p = 0; p <= d; ++p) {
         const int r = d - p;
-        acc[d] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as[p][0], bs[r][0], acc[d], 0, 0, 0);
-        acc[d] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as[p][1], bs[r][1], acc[d], 0, 0, 0);
+        acc[d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, as[p]),
+                                                         __builtin_bit_cast(v8bf, bs[r]), acc[d], 0, 0, 0);
       }
     }
     // row 4q+e of the result needs that row's exponent (held by lane 4q+e)
