@@ -413,7 +413,7 @@ def main():
                     help="measure what plain streaming kernels move (read, write, copy, 3 reads + 1 write) and exit")
     ap.add_argument("--serve-small", type=int, default=64,
                     help="requests of the population's smallest class through the serving loop (0: skip)")
-    ap.add_argument("--serve", type=int, default=8,
+    ap.add_argument("--serve", type=int, default=32,
                     help="requests of the serving loop to run beside the plain path (0: skip)")
     args = ap.parse_args()
     CFG["bc"] = args.bc

@@ -82,14 +82,16 @@ def stage_tile_planes(lm, tj=8):
     return (-(-(lm.ni - 2) // 64)) * (-(-(lm.Ny - 1) // tj)) * lm.Nz * nsys
 
 
-def fill_kc(members, candidates=(8, 16, 32), tj=8, min_workgroups=4096):
+def fill_kc(members, candidates=(8, 16, 32), tj=8, min_workgroups=1024):
     """The longest phi chunk that still leaves `min_workgroups` full-length
-    chunks of work per stage over the batch (16 per CU), else the shortest
+    chunks of work per stage over the batch (4 per CU), else the shortest
     candidate.  Longer chunks re-read fewer phi edges; shorter ones keep the
-    CUs busy when few or small grids are active.  On one MI355X the rule gives
-    the timed picks of `autotune` / `autotune_batch` among 8, 16 and 32 planes:
-    32 for one 2048 x 128 x 256 grid and for the 256-member uneven population,
-    16 for 256 grids of 256 x 16 x 32."""
+    CUs busy when few or small grids are active.  On one MI355X: 32 planes for
+    one 2048 x 128 x 256 grid (the plain autotuner's pick), and for the
+    served batch of 8 uneven-population requests (16 and 32 served them at
+    the one-at-a-time rate, a threshold of 16 chunks per CU, which chose 8
+    planes there, 4 % below it); the batch autotuner takes the longest chunk
+    for the 256-member population and for 256 grids of 256 x 16 x 32."""
     work = sum(stage_tile_planes(m, tj) for m in members)
     cands = sorted(candidates)
     for kc in reversed(cands):

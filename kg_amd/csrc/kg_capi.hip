@@ -15,7 +15,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <utility>
 #include <string>
@@ -46,8 +48,34 @@ struct RecOp {
   FillArgs f;
 };
 
+// A recorded batched step led by a handle: the launches of both parity
+// versions, their uploaded descriptors and (after the first replay) one
+// captured graph per version.  Valid while the same members, in the same
+// order, with the same tuning and parities step together, and while the
+// lead's descriptor buffer still holds its upload (gen).
+struct BatchPlan {
+  std::vector<int64_t> sig;
+  int n = 0;
+  size_t nops = 0;
+  std::vector<int> kind, mode;
+  std::vector<int> total;  // per (version, launch): workgroups of the stage / column launch
+  int nblocks[3] = {0, 0, 0}, wide[3] = {0, 0, 0};
+  size_t o_fd = 0, o_ts = 0, o_bs = 0, o_ni = 0;
+  char* dev = nullptr;
+  uint64_t gen = 0;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  BatchPlan() = default;
+  BatchPlan(BatchPlan&&) = default;
+  BatchPlan& operator=(BatchPlan&&) = default;
+  ~BatchPlan() {
+    for (auto& e : exec)
+      if (e) (void)hipGraphExecDestroy(e);
+  }
+};
+
 struct kg_handle {
   kg_desc desc{};
+  uint64_t serial = 0;  // unique per handle in the process (identifies members in cached batch plans)
   HostGrid hg{};
   Geom G{};
   int nsys = 1;
@@ -77,6 +105,8 @@ struct kg_handle {
   int64_t prof_n[3] = {0, 0, 0};
   void* batch_dev = nullptr;  // descriptors of batched launches led by this handle
   size_t batch_cap = 0;
+  uint64_t batch_gen = 0;     // uploads into batch_dev so far
+  std::unique_ptr<BatchPlan> plan;  // the last batched step this handle led
   mutable void* reduce_buf = nullptr;  // reductions of the diagnostics, kept between calls
   mutable size_t reduce_bytes = 0;
   hipEvent_t batch_done = nullptr;
@@ -517,115 +547,146 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
     }
     if (h != lead) KG_HIP(lead, hipStreamSynchronize(h->stream));
   }
-  // two consecutive steps of every member: version v is the step taken after
-  // v parity advances (each parity has period 2 under advance_parity)
-  std::vector<std::vector<RecOp>> rec(2 * (size_t)n);
+  // A plan (recorded launches and uploaded descriptors) is kept on the lead
+  // handle and reused, with its two step graphs, while the same members step
+  // together with the same tuning and parities.
+  std::vector<int64_t> sig;
+  sig.reserve(4 + 6 * (size_t)n);
+  sig.push_back(n);
+  sig.push_back(lead->tune_tj);
   for (int q = 0; q < n; ++q) {
-    kg_handle* h = hs[q];
-    const State5 u0 = h->u, a0 = h->A;
-    const int px0 = h->px, p50 = h->p5, p250 = h->p25;
-    kg_status st = KG_OK;
-    for (int v = 0; v < 2 && st == KG_OK; ++v) {
-      h->rec = &rec[(size_t)v * n + q];
-      st = enqueue_step(h, scheme);
-      h->rec = nullptr;
-      advance_parity(h, scheme);
-    }
-    h->u = u0;
-    h->A = a0;
-    h->px = px0;
-    h->p5 = p50;
-    h->p25 = p250;
-    if (st != KG_OK) return st;
+    const kg_handle* h = hs[q];
+    sig.push_back((int64_t)h->serial);
+    sig.push_back(h->tune_kc);
+    sig.push_back(h->px);
+    sig.push_back(h->p5);
+    sig.push_back(h->p25);
   }
-  const size_t nops = rec[0].size();
-  for (size_t r = 0; r < rec.size(); ++r) {
-    if (rec[r].size() != nops) return fail(lead, KG_ERR_INVALID, "internal: batch step shapes differ");
-    for (size_t o = 0; o < nops; ++o)
-      if (rec[r][o].kind != rec[0][o].kind || rec[r][o].mode != rec[0][o].mode)
-        return fail(lead, KG_ERR_INVALID, "internal: batch step launches differ");
-  }
-  // descriptors: stage args and fill args per (version, launch, member), the
-  // workgroup prefix of each stage launch, the fill passes' block prefixes
-  std::vector<StageArgs> sd(2 * nops * n);
-  std::vector<FillArgs> fd(2 * nops * n);
-  std::vector<int> tstart(2 * nops * (n + 1), 0), total(2 * nops, 0), bstart(3 * (n + 1));
-  std::vector<int64_t> nitems(3 * n);
-  for (int v = 0; v < 2; ++v)
-    for (size_t o = 0; o < nops; ++o) {
-      const size_t vo = (size_t)v * nops + o;
-      for (int q = 0; q < n; ++q) {
-        const RecOp& op = rec[(size_t)v * n + q][o];
-        if (op.kind == 0) {
-          StageArgs a = op.a;  // the member's own phi chunk; theta rows are the lead's
-          a.tj = lead->G.fd4 ? 8 : stage_tj(op.mode, lead->tune_tj);
-          finalize_tiles(a);
-          sd[vo * n + q] = a;
-          tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + a.nti * a.ntj * a.ntk * a.nsys;
-        } else if (op.kind == 2) {
-          // Sommerfeld column: blocks of somm_block() work items
-          sd[vo * n + q] = op.a;
-          const int64_t nb = (somm_items(op.a) + somm_block() - 1) / somm_block();
-          tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + (int)nb;
-        } else {
-          fd[vo * n + q] = op.f;
-        }
-      }
-      total[vo] = tstart[vo * (n + 1) + n];
-    }
-  int wide[3] = {0, 0, 0};  // a pass with 2^32 items or more in some grid
-  for (int ph = 0; ph < 3; ++ph) {
-    bstart[ph * (n + 1)] = 0;
+  const bool use_graphs = !noalloc && !lead->prof;
+  BatchPlan local;
+  BatchPlan* plan = nullptr;
+  if (!noalloc && lead->plan && lead->plan->sig == sig && lead->plan->gen == lead->batch_gen) plan = lead->plan.get();
+  if (!plan) {
+    plan = &local;
+    plan->sig = sig;
+    plan->n = n;
+    // two consecutive steps of every member: version v is the step taken after
+    // v parity advances (each parity has period 2 under advance_parity)
+    std::vector<std::vector<RecOp>> rec(2 * (size_t)n);
     for (int q = 0; q < n; ++q) {
-      const int64_t it = fill_work_items(hs[q]->G, hs[q]->nsys, ph);
-      nitems[ph * n + q] = it;
-      if (it >= (int64_t{1} << 32)) wide[ph] = 1;
-      bstart[ph * (n + 1) + q + 1] = bstart[ph * (n + 1) + q] + (int)((it + 255) / 256);
+      kg_handle* h = hs[q];
+      const State5 u0 = h->u, a0 = h->A;
+      const int px0 = h->px, p50 = h->p5, p250 = h->p25;
+      kg_status st = KG_OK;
+      for (int v = 0; v < 2 && st == KG_OK; ++v) {
+        h->rec = &rec[(size_t)v * n + q];
+        st = enqueue_step(h, scheme);
+        h->rec = nullptr;
+        advance_parity(h, scheme);
+      }
+      h->u = u0;
+      h->A = a0;
+      h->px = px0;
+      h->p5 = p50;
+      h->p25 = p250;
+      if (st != KG_OK) return st;
     }
-  }
-  // one device block: [sd][fd][tstart][bstart][nitems]
-  const size_t o_fd = sd.size() * sizeof(StageArgs);
-  const size_t o_ts = o_fd + fd.size() * sizeof(FillArgs);
-  const size_t o_bs = o_ts + tstart.size() * sizeof(int);
-  const size_t o_ni = (o_bs + bstart.size() * sizeof(int) + 7) / 8 * 8;
-  const size_t bytes = o_ni + nitems.size() * sizeof(int64_t);
-  std::vector<char> host(bytes);
-  memcpy(host.data(), sd.data(), o_fd);
-  memcpy(host.data() + o_fd, fd.data(), fd.size() * sizeof(FillArgs));
-  memcpy(host.data() + o_ts, tstart.data(), tstart.size() * sizeof(int));
-  memcpy(host.data() + o_bs, bstart.data(), bstart.size() * sizeof(int));
-  memcpy(host.data() + o_ni, nitems.data(), nitems.size() * sizeof(int64_t));
-  char* dev = nullptr;
-  if (noalloc) {
-    if (bytes > kBatchScratchBytes)
-      return fail(lead, KG_ERR_INVALID,
-                  "batch without allocation: %zu bytes of launch descriptors exceed its %zu-byte device array "
-                  "(fewer members per call)", bytes, kBatchScratchBytes);
-    dev = (char*)scratch_of(lead->device);
-    if (!dev) return fail(lead, KG_ERR_INVALID, "batch without allocation: device %d not set up", lead->device);
-  } else {
-    if (lead->batch_cap < bytes) {
-      KG_HIP(lead, hipStreamSynchronize(lead->stream));
-      if (lead->batch_dev) KG_HIP(lead, hipFree(lead->batch_dev));
-      lead->batch_dev = nullptr;
-      lead->batch_cap = 0;
-      KG_HIP(lead, hipMalloc(&lead->batch_dev, bytes));
-      lead->batch_cap = bytes;
+    const size_t nops = rec[0].size();
+    for (size_t r = 0; r < rec.size(); ++r) {
+      if (rec[r].size() != nops) return fail(lead, KG_ERR_INVALID, "internal: batch step shapes differ");
+      for (size_t o = 0; o < nops; ++o)
+        if (rec[r][o].kind != rec[0][o].kind || rec[r][o].mode != rec[0][o].mode)
+          return fail(lead, KG_ERR_INVALID, "internal: batch step launches differ");
     }
-    dev = (char*)lead->batch_dev;
-  }
-  KG_HIP(lead, hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, lead->stream));
-  const StageArgs* dsd = (const StageArgs*)dev;
-  const FillArgs* dfd = (const FillArgs*)(dev + o_fd);
-  const int* dts = (const int*)(dev + o_ts);
-  const int* dbs = (const int*)(dev + o_bs);
-  const int64_t* dni = (const int64_t*)(dev + o_ni);
-  for (int it = 0; it < nsteps; ++it) {
-    const int v = it & 1;
+    plan->nops = nops;
     for (size_t o = 0; o < nops; ++o) {
-      const size_t vo = (size_t)v * nops + o;
-      const int kind = rec[0][o].kind;
-      const bool prof = lead->prof && !noalloc;
+      plan->kind.push_back(rec[0][o].kind);
+      plan->mode.push_back(rec[0][o].mode);
+    }
+    // descriptors: stage args and fill args per (version, launch, member), the
+    // workgroup prefix of each stage launch, the fill passes' block prefixes
+    std::vector<StageArgs> sd(2 * nops * n);
+    std::vector<FillArgs> fd(2 * nops * n);
+    std::vector<int> tstart(2 * nops * (n + 1), 0), bstart(3 * (n + 1));
+    std::vector<int64_t> nitems(3 * n);
+    plan->total.assign(2 * nops, 0);
+    for (int v = 0; v < 2; ++v)
+      for (size_t o = 0; o < nops; ++o) {
+        const size_t vo = (size_t)v * nops + o;
+        for (int q = 0; q < n; ++q) {
+          const RecOp& op = rec[(size_t)v * n + q][o];
+          if (op.kind == 0) {
+            StageArgs a = op.a;  // the member's own phi chunk; theta rows are the lead's
+            a.tj = lead->G.fd4 ? 8 : stage_tj(op.mode, lead->tune_tj);
+            finalize_tiles(a);
+            sd[vo * n + q] = a;
+            tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + a.nti * a.ntj * a.ntk * a.nsys;
+          } else if (op.kind == 2) {
+            // Sommerfeld column: blocks of somm_block() work items
+            sd[vo * n + q] = op.a;
+            const int64_t nb = (somm_items(op.a) + somm_block() - 1) / somm_block();
+            tstart[vo * (n + 1) + q + 1] = tstart[vo * (n + 1) + q] + (int)nb;
+          } else {
+            fd[vo * n + q] = op.f;
+          }
+        }
+        plan->total[vo] = tstart[vo * (n + 1) + n];
+      }
+    for (int ph = 0; ph < 3; ++ph) {  // a pass with 2^32 items or more in some grid
+      plan->wide[ph] = 0;
+      bstart[ph * (n + 1)] = 0;
+      for (int q = 0; q < n; ++q) {
+        const int64_t it = fill_work_items(hs[q]->G, hs[q]->nsys, ph);
+        nitems[ph * n + q] = it;
+        if (it >= (int64_t{1} << 32)) plan->wide[ph] = 1;
+        bstart[ph * (n + 1) + q + 1] = bstart[ph * (n + 1) + q] + (int)((it + 255) / 256);
+      }
+      plan->nblocks[ph] = bstart[ph * (n + 1) + n];
+    }
+    // one device block: [sd][fd][tstart][bstart][nitems]
+    plan->o_fd = sd.size() * sizeof(StageArgs);
+    plan->o_ts = plan->o_fd + fd.size() * sizeof(FillArgs);
+    plan->o_bs = plan->o_ts + tstart.size() * sizeof(int);
+    plan->o_ni = (plan->o_bs + bstart.size() * sizeof(int) + 7) / 8 * 8;
+    const size_t bytes = plan->o_ni + nitems.size() * sizeof(int64_t);
+    std::vector<char> host(bytes);
+    memcpy(host.data(), sd.data(), plan->o_fd);
+    memcpy(host.data() + plan->o_fd, fd.data(), fd.size() * sizeof(FillArgs));
+    memcpy(host.data() + plan->o_ts, tstart.data(), tstart.size() * sizeof(int));
+    memcpy(host.data() + plan->o_bs, bstart.data(), bstart.size() * sizeof(int));
+    memcpy(host.data() + plan->o_ni, nitems.data(), nitems.size() * sizeof(int64_t));
+    if (noalloc) {
+      if (bytes > kBatchScratchBytes)
+        return fail(lead, KG_ERR_INVALID,
+                    "batch without allocation: %zu bytes of launch descriptors exceed its %zu-byte device array "
+                    "(fewer members per call)", bytes, kBatchScratchBytes);
+      plan->dev = (char*)scratch_of(lead->device);
+      if (!plan->dev) return fail(lead, KG_ERR_INVALID, "batch without allocation: device %d not set up", lead->device);
+    } else {
+      if (lead->batch_cap < bytes) {
+        KG_HIP(lead, hipStreamSynchronize(lead->stream));
+        if (lead->batch_dev) KG_HIP(lead, hipFree(lead->batch_dev));
+        lead->batch_dev = nullptr;
+        lead->batch_cap = 0;
+        KG_HIP(lead, hipMalloc(&lead->batch_dev, bytes));
+        lead->batch_cap = bytes;
+      }
+      plan->dev = (char*)lead->batch_dev;
+      plan->gen = ++lead->batch_gen;  // an older plan's descriptors are overwritten now
+    }
+    KG_HIP(lead, hipMemcpyAsync(plan->dev, host.data(), bytes, hipMemcpyHostToDevice, lead->stream));
+  }
+  // the launches of one step of version v (0: the step the members take now)
+  const BatchPlan& P = *plan;
+  const StageArgs* dsd = (const StageArgs*)P.dev;
+  const FillArgs* dfd = (const FillArgs*)(P.dev + P.o_fd);
+  const int* dts = (const int*)(P.dev + P.o_ts);
+  const int* dbs = (const int*)(P.dev + P.o_bs);
+  const int64_t* dni = (const int64_t*)(P.dev + P.o_ni);
+  auto launch_version = [&](int v, bool prof) -> kg_status {
+    for (size_t o = 0; o < P.nops; ++o) {
+      const size_t vo = (size_t)v * P.nops + o;
+      const int kind = P.kind[o], mode = P.mode[o];
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (prof) {
         e0 = next_event(lead);
@@ -634,21 +695,45 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
         KG_HIP(lead, hipEventRecord(e0, lead->stream));
       }
       if (kind == 0) {
-        KG_HIP(lead, launch_stage_batch(rec[0][o].mode, dsd + vo * n, dts + vo * (n + 1), n, total[vo],
-                                        stage_tj(rec[0][o].mode, lead->tune_tj), ef, lead->G.fd4, lead->stream));
+        KG_HIP(lead, launch_stage_batch(mode, dsd + vo * n, dts + vo * (n + 1), n, P.total[vo],
+                                        stage_tj(mode, lead->tune_tj), ef, lead->G.fd4, lead->stream));
       } else if (kind == 2) {
-        KG_HIP(lead, launch_somm_batch(rec[0][o].mode, dsd + vo * n, dts + vo * (n + 1), n, total[vo], lead->G.fd4,
+        KG_HIP(lead, launch_somm_batch(mode, dsd + vo * n, dts + vo * (n + 1), n, P.total[vo], lead->G.fd4,
                                        lead->stream));
       } else {
         for (int ph = 0; ph < 3; ++ph)
-          KG_HIP(lead, launch_fill_batch(dfd + vo * n, dbs + ph * (n + 1), n, dni + ph * n,
-                                         bstart[ph * (n + 1) + n], ph, wide[ph], lead->stream));
+          KG_HIP(lead, launch_fill_batch(dfd + vo * n, dbs + ph * (n + 1), n, dni + ph * n, P.nblocks[ph], ph,
+                                         P.wide[ph], lead->stream));
       }
       if (prof) {
         KG_HIP(lead, hipEventRecord(e1, lead->stream));
         lead->ev_marks.push_back({kind, lead->ev_used - 2});
       }
     }
+    return KG_OK;
+  };
+  if (use_graphs) {
+    for (int v = 0; v < 2 && v < nsteps; ++v) {
+      if (plan->exec[v]) continue;
+      hipGraph_t graph;
+      KG_HIP(lead, hipStreamBeginCapture(lead->stream, hipStreamCaptureModeThreadLocal));
+      const kg_status st = launch_version(v, false);
+      const hipError_t ec = hipStreamEndCapture(lead->stream, &graph);
+      if (st != KG_OK) return st;
+      if (ec != hipSuccess) return fail(lead, KG_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&plan->exec[v], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return fail(lead, KG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    }
+    for (int it = 0; it < nsteps; ++it) KG_HIP(lead, hipGraphLaunch(plan->exec[it & 1], lead->stream));
+  } else {
+    const bool prof = lead->prof && !noalloc;
+    for (int it = 0; it < nsteps; ++it) KG_TRY(launch_version(it & 1, prof));
+  }
+  if (plan == &local && !noalloc) {
+    // keep the plan (and its graphs) for the next call with the same members
+    lead->plan.reset(new BatchPlan(std::move(local)));
+    local.exec[0] = local.exec[1] = nullptr;
   }
   // bookkeeping and ordering: later work on any member waits for the batch
   // (the event is made at kg_create)
@@ -696,7 +781,9 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, KG_ERR_NO_DEVICE, "no GPU visible");
 
+  static std::atomic<uint64_t> next_serial{1};
   kg_handle* h = new kg_handle();
+  h->serial = next_serial++;
   h->desc = *d;
   h->nsys = d->is_complex ? 2 : 1;
   build_grid(d->Nx, d->Ny, d->Nz, d->g_pts, d->xmin, d->xmax, d->CFL, d->dt, &h->hg);
@@ -777,6 +864,7 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
 kg_status kg_destroy(kg_handle* h) {
   if (!h) return KG_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->plan.reset();
   for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);

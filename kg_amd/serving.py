@@ -173,7 +173,7 @@ class GpuEngine:
     projector; ``step`` is one batched launch per RK4 stage."""
 
     def __init__(self, lmax: int = 8, nk: int = 512, kmax: float = 4.0, CFL: float = 0.25, tuning=None,
-                 on_device: bool = True, operators: str = "ref2", batch_kc=None, fill_workgroups: int = 4096,
+                 on_device: bool = True, operators: str = "ref2", batch_kc=None, fill_workgroups: int = 1024,
                  scheme: str = "rk4"):
         """tuning: (kc, tj) of a request run alone; batch_kc: the phi chunk of
         requests advanced together (defaults to tuning's), or "fill" to pick it

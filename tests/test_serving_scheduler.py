@@ -134,9 +134,10 @@ def test_poisson_stream_and_percentiles():
 
 
 def test_fill_rule_phi_chunks():
-    """The workgroup-fill rule: the longest phi chunk leaving 4096 full-length
-    chunks per stage; measured picks it reproduces (one 2048x128x256 grid: 32;
-    256 grids of 256x16x32, 33 phi planes each: 16) and its fallback."""
+    """The workgroup-fill rule: the longest phi chunk leaving 1024 full-length
+    chunks per stage (4 per CU): one 2048x128x256 grid takes 32 (the plain
+    autotuner's pick), 256 grids of 256x16x32 (33 phi planes each) the
+    longest candidate, and its fallback."""
     from types import SimpleNamespace
 
     from kg_amd.autotune import fill_kc, stage_tile_planes
@@ -148,7 +149,8 @@ def test_fill_rule_phi_chunks():
     assert stage_tile_planes(big) == 33 * 16 * 257
     assert stage_tile_planes(grid(2048, 128, 256, True)) == 2 * 33 * 16 * 257
     assert fill_kc([big]) == 32
-    assert fill_kc([grid(256, 16, 32)] * 256) == 16
+    assert fill_kc([grid(256, 16, 32)] * 256) == 32
+    assert fill_kc([grid(256, 16, 32)] * 60) == 16  # 19,800 tile planes
     assert fill_kc([grid(256, 16, 32)]) == 8  # nothing fills: the shortest
     assert fill_kc([big] * 4, candidates=(8, 16, 32, 64)) == 64
     assert fill_kc([grid(24, 8, 16)], min_workgroups=1) == 16  # 17 planes: a 32 chunk would leave < 1
