@@ -61,7 +61,7 @@ def cpu_baseline(operators, max_seconds=12.0):
     from kg_amd.initial_data import shell_state, random_lm_coeffs
 
     Nx, Ny, Nz = CFG["Nx"], CFG["Ny"], 17
-    sim = oracle.Sim(Nx, Ny, Nz, 2, CFG["xmin"], CFG["xmax"], "Kerr_Schild", 0.0, 1.0, CFG["m"], "outflow",
+    sim = oracle.Sim(Nx, Ny, Nz, 2, CFG["xmin"], CFG["xmax"], "Kerr_Schild", 0.0, 1.0, CFG["m"], CFG["bc"],
                      CFG["CFL"])
     sim.set_order(4 if operators == "fd4" else 2)
     sim.u[...] = shell_state(Nx, Ny, Nz, 2, CFG["xmin"], CFG["xmax"], 30.0, 3.0, 1.0, random_lm_coeffs(20261203))
@@ -74,10 +74,13 @@ def cpu_baseline(operators, max_seconds=12.0):
             break
     ups = Nx * (Ny - 1) * (Nz - 1) * steps / el
     nt = oracle.threads()
+    full = CFG["Nx"] * (CFG["Ny"] - 1) * (CFG["Nz"] - 1)
     return {"value": ups, "unit": "grid-point updates/s", "cores": nt, "kind": "port",
-            "sample": "oracle/kg_oracle.c RK4 (%s operators), %d steps on Nr=%d, Ntheta=%d, Nphi=%d (the workload's "
-                      "r-theta grid, 16 phi planes), %d OpenMP threads, %.1f s" % (operators, steps, Nx, Ny - 1, Nz - 1,
-                                                                                  nt, el)}
+            "sample": "oracle/kg_oracle.c RK4 (%s operators, %s outer boundary), %d steps on Nr=%d, Ntheta=%d, "
+                      "Nphi=%d (the workload's r-theta grid, 16 phi planes), %d OpenMP threads, %.1f s; the work per "
+                      "point update is the same on every phi plane, so at this rate one step of the full %d-point "
+                      "grid takes %.2f s" % (operators, CFG["bc"], steps, Nx, Ny - 1, Nz - 1, nt, el, full,
+                                             full / ups)}
 
 
 TRAFFIC_PROFILES = {"ref2": "stage_kernel_rk4_nr2048_traffic.json", "fd4": "stage4_kernel_rk4_nr2048_traffic.json"}

@@ -35,14 +35,16 @@ def _rel_err(a, b):
 
 def test_full_workload_physics_reduced_grid_200_steps():
     """Nr = 512, Ntheta = 32, Nphi = 64 on r in [1.8, 200] with the full
-    workload's mass, pulse and operators: bit-identical after 200 RK4 steps."""
+    workload's mass, pulse, operators and Sommerfeld outer boundary:
+    bit-identical after 200 RK4 steps."""
     Nx, Ny, Nz = 512, 33, 65
     print("full-workload physics on 512 x 32 x 64, fd4 RK4, 200 steps", flush=True)
-    sim = oracle.Sim(Nx, Ny, Nz, 2, 1.8, 200.0, "Kerr_Schild", 0.0, 1.0, 0.1, "outflow", 0.25)
+    sim = oracle.Sim(Nx, Ny, Nz, 2, 1.8, 200.0, "Kerr_Schild", 0.0, 1.0, 0.1, "sommerfeld", 0.25)
     sim.set_order(4)
     u0 = sim.fill(shell_state(Nx, Ny, Nz, 2, 1.8, 200.0, 30.0, 3.0, 1.0, random_lm_coeffs(20261203)))
     sim.u[...] = u0
-    gpu = LinesMethod(Nx=Nx, Ny=Ny, Nz=Nz, xmin=1.8, xmax=200.0, CFL=0.25, m=0.1, scheme="rk4", operators="fd4")
+    gpu = LinesMethod(Nx=Nx, Ny=Ny, Nz=Nz, xmin=1.8, xmax=200.0, CFL=0.25, m=0.1, scheme="rk4", operators="fd4",
+                      boundary_type_rmax="sommerfeld")
     assert gpu.dt == sim.dt
     gpu.set_u_soa(u0[None])
     sim.step("rk4", 200)
@@ -55,7 +57,8 @@ def test_full_workload_physics_reduced_grid_200_steps():
 
 
 def test_full_workload_full_size_is_shape_and_batch_invariant():
-    """2048 x 128 x 256, fd4 RK4, m = 0.1: 20 steps at the default launch shape,
+    """2048 x 128 x 256, fd4 RK4, m = 0.1, Sommerfeld outer boundary: 20 steps
+    at the default launch shape,
     at two others and as a one-member batched launch give the same state bit
     for bit, finite and nonzero."""
     import torch
@@ -70,7 +73,7 @@ def test_full_workload_full_size_is_shape_and_batch_invariant():
     for label, kc in (("default", None), ("kc=8", 8), ("kc=64", 64), ("batched", None)):
         print("full size, 20 steps,", label, flush=True)
         lm = LinesMethod(Nx=Nx, Ny=Ny, Nz=Nz, xmin=1.8, xmax=200.0, CFL=0.25, m=0.1, scheme="rk4",
-                         operators="fd4")
+                         operators="fd4", boundary_type_rmax="sommerfeld")
         lm.copy_state_device(u0.data_ptr(), to_handle=True)
         lm.boundary()
         if kc is not None:
