@@ -173,33 +173,47 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
 def spectral_projection(members, batched_step_s):
     """The every-50-steps projection of the batched workload: phi of every member
     onto j_l(k r), l <= 8, k_n = 4n/512 (n = 1..512), split-BF16 matrix cores
-    and FP64, all members back to back on the GPU (results left in HBM)."""
+    and FP64, results left in HBM: all members in one launch per pass
+    (kg_spec_radial_batch), and the same members one projection call each
+    (kg_spec_radial on every member's own stream), whose results it must
+    equal bit for bit."""
     import numpy as np
     import torch
-    from kg_amd.spectral import SpectralProjector
+    from kg_amd.spectral import SpectralProjector, radial_batch
 
     lmax, nk = 8, 512
     projs = [SpectralProjector(m, lmax=lmax, nk=nk, basis="uniform", kmax=4.0) for m in members]
     nlm = (lmax + 1) ** 2
-    outs = {p: torch.empty((len(projs), nlm * nk * 2), dtype=torch.float64, device="cuda")
-            for p in ("split_bf16", "fp64")}
+    outs = {(p, mode): torch.empty((len(projs), nlm * nk * 2), dtype=torch.float64, device="cuda")
+            for p in ("split_bf16", "fp64") for mode in ("batch", "each")}
     times = {}
-    for prec in ("split_bf16", "fp64", "split_bf16", "fp64"):  # the first pass of each warms up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for q, sp in enumerate(projs):
-            sp.radial(0, prec, out_device_ptr=outs[prec][q].data_ptr())
-        for m in members:
-            m.sync()
-        times[prec] = time.perf_counter() - t0
-    a, b = outs["split_bf16"].cpu().numpy(), outs["fp64"].cpu().numpy()
+    for rep in range(2):  # the first pass of each warms up
+        for prec in ("split_bf16", "fp64"):
+            for mode in ("batch", "each"):
+                o = outs[(prec, mode)]
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if mode == "batch":
+                    radial_batch(projs, 0, prec, out_device_ptrs=[o[q].data_ptr() for q in range(len(projs))])
+                    members[0].sync()
+                else:
+                    for q, sp in enumerate(projs):
+                        sp.radial(0, prec, out_device_ptr=o[q].data_ptr())
+                    for m in members:
+                        m.sync()
+                times[(prec, mode)] = time.perf_counter() - t0
+    same = all(torch.equal(outs[(p, "batch")], outs[(p, "each")]) for p in ("split_bf16", "fp64"))
+    a, b = outs[("split_bf16", "batch")].cpu().numpy(), outs[("fp64", "batch")].cpu().numpy()
     err = max(float(np.abs(a[q] - b[q]).max() / max(np.abs(b[q]).max(), 1e-300)) for q in range(len(projs)))
     for sp in projs:
         sp.close()
     return {"lmax": lmax, "nk": nk, "members": len(projs),
-            "ms_split_bf16": times["split_bf16"] * 1e3, "ms_fp64": times["fp64"] * 1e3,
+            "ms_split_bf16": times[("split_bf16", "batch")] * 1e3, "ms_fp64": times[("fp64", "batch")] * 1e3,
+            "ms_split_bf16_one_call_per_member": times[("split_bf16", "each")] * 1e3,
+            "ms_fp64_one_call_per_member": times[("fp64", "each")] * 1e3,
+            "batched_equals_per_member": same,
             "max_rel_err_split_bf16_vs_fp64": err,
-            "fraction_of_50_steps": times["split_bf16"] / (50.0 * batched_step_s)}
+            "fraction_of_50_steps": times[("split_bf16", "batch")] / (50.0 * batched_step_s)}
 
 
 def initial_data(ncart, reps=3):

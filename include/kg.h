@@ -247,6 +247,14 @@ kg_status kg_spec_project_sh(kg_spec* p, int32_t field, double* flm, int32_t on_
  * ft(n,l,m) = trapezoid_i( flm(r_i) j_l(k_n r_i) r_i^2, dr ), computed from
  * domain_to_sph + project_sh of `field`: ft = complex [nlm][nk]. */
 kg_status kg_spec_radial(kg_spec* p, int32_t field, int32_t precision, double* ft, int32_t on_device);
+/* kg_spec_radial of many projectors (grids of any size, one lmax class:
+ * all lmax <= 8 or all > 8), one launch per pass (phi, theta, radial) over
+ * all of them on ps[0]'s stream; fts[q] receives projector q's complex
+ * [nlm][nk].  Results equal kg_spec_radial on each projector bit for bit.
+ * No counterpart in the reference (one grid per process); its per-grid
+ * routine is SphFT_discrete_radial (FFT_sph.f90:213-244). */
+kg_status kg_spec_radial_batch(kg_spec* const* ps, int32_t n, int32_t field, int32_t precision, double* const* fts,
+                               int32_t on_device);
 /* The radial contraction alone, on caller-supplied host flm (complex
  * [nlm][Nx]) -> host ft (complex [nlm][nk]). */
 kg_status kg_spec_radial_flm(kg_spec* p, const double* flm, int32_t precision, double* ft);

@@ -97,6 +97,8 @@ SIGNATURES = {
     "kg_spec_project_sh": (C.c_int32, [_vp, C.c_int32, _vp, C.c_int32]),
     "kg_spec_radial": (C.c_int32, [_vp, C.c_int32, C.c_int32, _vp, C.c_int32]),
     "kg_spec_radial_flm": (C.c_int32, [_vp, _vp, C.c_int32, _vp]),
+    "kg_spec_radial_batch": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32,
+                                         C.POINTER(C.c_void_p), C.c_int32]),
     "kg_spec_detectors": (C.c_int32, [_vp, C.c_int32, _vp, _vp, _vp]),
     "kg_diag_integrate": (C.c_int32, [_vp, C.c_int32, _dp]),
     "kg_diag_last_error": (C.c_char_p, []),

@@ -64,21 +64,31 @@ struct InterpArgs {
   int Nr, Nt, Np;
 };
 
+// The projection reads its tables and the field through address-space
+// qualified pointers: the field and the per-radius tables as global memory,
+// the per-theta and per-phi tables and the phi factors (read-only for the
+// whole launch, the same address across a wave) as constant memory, which
+// lets those loads go through the scalar unit instead of 64 vector lanes.
+typedef __attribute__((address_space(1))) const double gcd;
+typedef __attribute__((address_space(1))) const int gci;
+typedef __attribute__((address_space(4))) const double kcd;
+typedef __attribute__((address_space(4))) const int kci;
+
 __device__ __forceinline__ double interp(const InterpArgs& a, const double* __restrict__ F, int i, int j, int k) {
-  const int ir = a.ir[i];
-  const int tp = a.tpole[j];
-  const int kp = a.iph[k];
-  auto at = [&](int ii, int jj, int kk) { return F[(int64_t)kk * a.plane + (int64_t)jj * a.ld + ii]; };
+  const int ir = ((gci*)a.ir)[i];
+  const int tp = ((kci*)a.tpole)[j];
+  const int kp = ((kci*)a.iph)[k];
+  auto at = [&](int ii, int jj, int kk) { return ((gcd*)F)[(int64_t)kk * a.plane + (int64_t)jj * a.ld + ii]; };
   if (tp == 1) return at(ir, 0, kp);
   if (tp == 2) return at(ir, a.nj - 1, kp);
-  if (a.pskip[k]) return 0.0;
-  const int jt = a.ith[j];
+  if (((kci*)a.pskip)[k]) return 0.0;
+  const int jt = ((kci*)a.ith)[j];
   const int k1 = kp + 1;
   const double f000 = at(ir, jt, kp), f001 = at(ir, jt, k1);
   const double f010 = at(ir, jt + 1, kp), f011 = at(ir, jt + 1, k1);
   const double f100 = at(ir + 1, jt, kp), f101 = at(ir + 1, jt, k1);
   const double f110 = at(ir + 1, jt + 1, kp), f111 = at(ir + 1, jt + 1, k1);
-  const double w_r = a.wr[i], w_th = a.wth[j], w_ph = a.wph[k];
+  const double w_r = ((gcd*)a.wr)[i], w_th = ((kcd*)a.wth)[j], w_ph = ((kcd*)a.wph)[k];
   const double f00 = (1.0 - w_ph) * f000 + w_ph * f001;
   const double f01 = (1.0 - w_ph) * f010 + w_ph * f011;
   const double f10 = (1.0 - w_ph) * f100 + w_ph * f101;
@@ -100,14 +110,14 @@ __global__ __launch_bounds__(256) void d2s_kernel(const InterpArgs a, const doub
 }
 
 // cs[s][j][q][i], q = 2m (cos) / 2m+1 (sin), m < MC; trig[k][m][2] carries
-// the phi trapezoid weight.  grid (ceil(Nr/64), ceil(Nt/4), nsys), block 256
+// the phi trapezoid weight.  Blocks (bx, by, s) of a grid (ceil(Nr/64),
+// ceil(Nt/4), nsys), 256 threads
 template <int MC>
-__global__ __launch_bounds__(256) void phi_kernel(const InterpArgs a, const double* __restrict__ F,
-                                                  int64_t sys_stride, const double* __restrict__ trig,
-                                                  double* __restrict__ cs) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int s = blockIdx.z;
+__device__ __forceinline__ void phi_body(const InterpArgs& a, const double* __restrict__ F, int64_t sys_stride,
+                                         const double* __restrict__ trig, double* __restrict__ cs, int bx, int by,
+                                         int s) {
+  const int i = bx * 64 + (threadIdx.x & 63);
+  const int j = by * 4 + (threadIdx.x >> 6);
   if (i >= a.Nr || j >= a.Nt) return;
   const double* Fs = F + s * sys_stride;
   double acc[2 * MC];
@@ -115,42 +125,51 @@ __global__ __launch_bounds__(256) void phi_kernel(const InterpArgs a, const doub
   for (int q = 0; q < 2 * MC; ++q) acc[q] = 0.0;
   for (int k = 0; k < a.Np; ++k) {
     const double f = interp(a, Fs, i, j, k);
-    const double* tk = trig + (int64_t)k * 2 * MC;
+    kcd* tk = (kcd*)trig + (int64_t)k * 2 * MC;
 #pragma unroll
     for (int q = 0; q < 2 * MC; ++q) acc[q] += tk[q] * f;
   }
-  double* o = cs + (((int64_t)s * a.Nt + j) * 2 * MC) * a.Nr + i;
+  __attribute__((address_space(1))) double* o =
+      (__attribute__((address_space(1))) double*)cs + (((int64_t)s * a.Nt + j) * 2 * MC) * a.Nr + i;
 #pragma unroll
   for (int q = 0; q < 2 * MC; ++q) o[(int64_t)q * a.Nr] = acc[q];
+}
+
+template <int MC>
+__global__ __launch_bounds__(256) void phi_kernel(const InterpArgs a, const double* __restrict__ F,
+                                                  int64_t sys_stride, const double* __restrict__ trig,
+                                                  double* __restrict__ cs) {
+  phi_body<MC>(a, F, sys_stride, trig, cs, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // flm[lm][part][i] from cs; one thread per (i, l, m >= 0) writes (l, m) and
 // (l, -m).  leg[j][pair], pair = l(l+1)/2 + m.  grid (ceil(Nr/64),
 // ceil(npair/4)), block 256
-__global__ __launch_bounds__(256) void theta_kernel(const double* __restrict__ cs, const double* __restrict__ leg,
-                                                    double* __restrict__ flm, int Nr, int Nt, int MC, int lmax,
-                                                    int nsys) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int pair = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void theta_body(const double* __restrict__ cs, const double* __restrict__ leg,
+                                           double* __restrict__ flm, int Nr, int Nt, int MC, int lmax, int nsys,
+                                           int bx, int by) {
+  const int i = bx * 64 + (threadIdx.x & 63);
+  const int pair = by * 4 + (threadIdx.x >> 6);
   const int npair = (lmax + 1) * (lmax + 2) / 2;
   if (i >= Nr || pair >= npair) return;
   int l = 0;
   while ((l + 1) * (l + 2) / 2 <= pair) ++l;
   const int m = pair - l * (l + 1) / 2;
   const int64_t jstride = (int64_t)2 * MC * Nr;
-  const double* ca = cs + (int64_t)(2 * m) * Nr + i;
-  const double* sa = ca + Nr;
+  gcd* ca = (gcd*)cs + (int64_t)(2 * m) * Nr + i;
+  gcd* sa = ca + Nr;
+  kcd* lg = (kcd*)leg;
   double lca = 0.0, lsa = 0.0, lcb = 0.0, lsb = 0.0;
   if (nsys == 1) {
     for (int j = 0; j < Nt; ++j) {
-      const double L = leg[(int64_t)j * npair + pair];
+      const double L = lg[(int64_t)j * npair + pair];
       lca += L * ca[j * jstride];
       lsa += L * sa[j * jstride];
     }
   } else {
     const int64_t s1 = (int64_t)Nt * jstride;
     for (int j = 0; j < Nt; ++j) {
-      const double L = leg[(int64_t)j * npair + pair];
+      const double L = lg[(int64_t)j * npair + pair];
       lca += L * ca[j * jstride];
       lsa += L * sa[j * jstride];
       lcb += L * ca[s1 + j * jstride];
@@ -160,14 +179,21 @@ __global__ __launch_bounds__(256) void theta_kernel(const double* __restrict__ c
   // f conj(Y_lm):   m >= 0: (Ca + Sb) + i (Cb - Sa)
   //                 m <  0: (-1)^m [(Ca - Sb) + i (Cb + Sa)]   (Y_{l,-m} = (-1)^m conj Y_lm)
   const int lm = l * l + l + m;
-  flm[((int64_t)lm * 2 + 0) * Nr + i] = lca + lsb;
-  flm[((int64_t)lm * 2 + 1) * Nr + i] = lcb - lsa;
+  __attribute__((address_space(1))) double* out = (__attribute__((address_space(1))) double*)flm;
+  out[((int64_t)lm * 2 + 0) * Nr + i] = lca + lsb;
+  out[((int64_t)lm * 2 + 1) * Nr + i] = lcb - lsa;
   if (m > 0) {
     const double sg = (m & 1) ? -1.0 : 1.0;
     const int ln = l * l + l - m;
-    flm[((int64_t)ln * 2 + 0) * Nr + i] = sg * (lca - lsb);
-    flm[((int64_t)ln * 2 + 1) * Nr + i] = sg * (lcb + lsa);
+    out[((int64_t)ln * 2 + 0) * Nr + i] = sg * (lca - lsb);
+    out[((int64_t)ln * 2 + 1) * Nr + i] = sg * (lcb + lsa);
   }
+}
+
+__global__ __launch_bounds__(256) void theta_kernel(const double* __restrict__ cs, const double* __restrict__ leg,
+                                                    double* __restrict__ flm, int Nr, int Nt, int MC, int lmax,
+                                                    int nsys) {
+  theta_body(cs, leg, flm, Nr, Nt, MC, lmax, nsys, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------- j_l on the device
@@ -253,12 +279,12 @@ __global__ __launch_bounds__(256) void bessel_kernel(const double* __restrict__ 
 // ---------------------------------------------------------------- radial FP64
 // Block = (l, 64 wave numbers); 4 column groups of CPT columns each.
 template <int CPT>
-__global__ __launch_bounds__(256) void radial_f64_kernel(const int2* __restrict__ tiles, const double* __restrict__ W,
-                                                         const double* __restrict__ flm, double* __restrict__ ft,
-                                                         int Nr, int nk) {
+__device__ __forceinline__ void radial_f64_body(const int2* __restrict__ tiles, const double* __restrict__ W,
+                                                const double* __restrict__ flm, double* __restrict__ ft, int Nr,
+                                                int nk, int tile) {
   __shared__ double sW[64][65];
   __shared__ double sB[4 * CPT][64];
-  const int2 t = tiles[blockIdx.x];
+  const int2 t = tiles[tile];
   const int l = t.x, n0 = t.y;
   const int C = 2 * (2 * l + 1);
   const int lane = threadIdx.x & 63, cg = threadIdx.x >> 6;
@@ -291,6 +317,13 @@ __global__ __launch_bounds__(256) void radial_f64_kernel(const int2* __restrict_
   }
 }
 
+template <int CPT>
+__global__ __launch_bounds__(256) void radial_f64_kernel(const int2* __restrict__ tiles, const double* __restrict__ W,
+                                                         const double* __restrict__ flm, double* __restrict__ ft,
+                                                         int Nr, int nk) {
+  radial_f64_body<CPT>(tiles, W, flm, ft, Nr, nk, blockIdx.x);
+}
+
 // ---------------------------------------------------------------- radial split BF16
 typedef short v8s __attribute__((ext_vector_type(8)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -317,88 +350,186 @@ __device__ __forceinline__ void slice8(const double* v, v8s sl[kSlices]) {
   }
 }
 
-// One wave per 16 x 16 output tile {l, n0, c0}, one chunk of 32 radii per
-// 16x16x32 BF16 MFMA (gfx950).  Operand layout: lane L holds
+// One wave per {l, 16 wave numbers n0..n0+15}, covering every 16-column tile
+// of that l (the W rows are loaded and sliced once per chunk for all of
+// them); one chunk of 32 radii per 16x16x32 BF16 MFMA (gfx950).  Operand layout: lane L holds
 // A[L%16][8(L/16)+e], B[8(L/16)+e][L%16], e = 0..7, and the result
 // D[4(L/16)+e][L%16], e = 0..3.  A = W_l rows (wave numbers), B = flm columns.
 // Every partial sum of a diagonal is an integer below 2^24 units, so the FP32
 // accumulation is exact in any order and the result does not depend on how
 // the matrix core groups the 32 products.
-__global__ __launch_bounds__(256) void radial_mfma_kernel(const int4* __restrict__ tiles, int ntiles,
-                                                          const double* __restrict__ W,
-                                                          const double* __restrict__ flm, double* __restrict__ ft,
-                                                          int Nr, int nk) {
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+constexpr int kMaxCT = 5;  // 16-column tiles of one l: C = 2(2l+1) <= 66 for l <= 16
+
+__device__ __forceinline__ void radial_mfma_body(const int4* __restrict__ tiles, int ntiles,
+                                                 const double* __restrict__ W, const double* __restrict__ flm,
+                                                 double* __restrict__ ft, int Nr, int nk, int block) {
+  const int tile = block * 4 + (threadIdx.x >> 6);
   if (tile >= ntiles) return;  // whole wave
   const int4 t = tiles[tile];
-  const int l = t.x, n0 = t.y, c0 = t.z;
+  const int l = t.x, n0 = t.y;
   const int C = 2 * (2 * l + 1);
+  const int nct = (C + 15) / 16;  // column tiles, all handled by this wave: W is sliced once for them
   const int L = threadIdx.x & 63;
   const int q = L >> 4, rl = L & 15;
-  const int n = n0 + rl, c = c0 + rl;
-  const bool vn = n < nk, vc = c < C;
+  const int n = n0 + rl;
+  const bool vn = n < nk;
   const double* Arow = W + ((int64_t)l * nk + (vn ? n : 0)) * Nr;
-  const double* Bcol = flm + ((int64_t)2 * l * l + (vc ? c : 0)) * Nr;
-  double out[4] = {0.0, 0.0, 0.0, 0.0};
+  double out[kMaxCT][4];
+#pragma unroll
+  for (int ct = 0; ct < kMaxCT; ++ct)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[ct][e] = 0.0;
   for (int i0 = 0; i0 < Nr; i0 += kChunk) {
-    double a[8], b[8];
+    double a[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int i = i0 + 8 * q + e;
       a[e] = (vn && i < Nr) ? Arow[i] : 0.0;
-      b[e] = (vc && i < Nr) ? Bcol[i] : 0.0;
     }
-    double am = 0.0, bm = 0.0;
+    double am = 0.0;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      am = fmax(am, fabs(a[e]));
-      bm = fmax(bm, fabs(b[e]));
-    }
+    for (int e = 0; e < 8; ++e) am = fmax(am, fabs(a[e]));
     am = fmax(am, __shfl_xor(am, 16));
     am = fmax(am, __shfl_xor(am, 32));
-    bm = fmax(bm, __shfl_xor(bm, 16));
-    bm = fmax(bm, __shfl_xor(bm, 32));
-    int ea = 0, eb = 0;
+    int ea = 0;
     if (am > 0.0) frexp(am, &ea);  // am < 2^ea
-    if (bm > 0.0) frexp(bm, &eb);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      a[e] = ldexp(a[e], -ea);
-      b[e] = ldexp(b[e], -eb);
-    }
-    v8s as[kSlices], bs[kSlices];
+    for (int e = 0; e < 8; ++e) a[e] = ldexp(a[e], -ea);
+    v8s as[kSlices];
     slice8(a, as);
-    slice8(b, bs);
-    v4f acc[kSlices];
+    // row 4q+e of the result needs that row's exponent (held by lane 4q+e)
+    int er[4];
 #pragma unroll
-    for (int d = 0; d < kSlices; ++d) acc[d] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int e = 0; e < 4; ++e) er[e] = __shfl(ea, 4 * q + e);
 #pragma unroll
-    for (int d = 0; d < kSlices; ++d) {
+    for (int ct = 0; ct < kMaxCT; ++ct) {
+      if (ct >= nct) break;  // wave-uniform
+      const int c = 16 * ct + rl;
+      const bool vc = c < C;
+      const double* Bcol = flm + ((int64_t)2 * l * l + (vc ? c : 0)) * Nr;
+      double b[8];
 #pragma unroll
-      for (int p = 0; p <= d; ++p) {
-        const int r = d - p;
-        acc[d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, as[p]),
-                                                         __builtin_bit_cast(v8bf, bs[r]), acc[d], 0, 0, 0);
+      for (int e = 0; e < 8; ++e) {
+        const int i = i0 + 8 * q + e;
+        b[e] = (vc && i < Nr) ? Bcol[i] : 0.0;
+      }
+      double bm = 0.0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bm = fmax(bm, fabs(b[e]));
+      bm = fmax(bm, __shfl_xor(bm, 16));
+      bm = fmax(bm, __shfl_xor(bm, 32));
+      int eb = 0;
+      if (bm > 0.0) frexp(bm, &eb);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) b[e] = ldexp(b[e], -eb);
+      v8s bs[kSlices];
+      slice8(b, bs);
+      v4f acc[kSlices];
+#pragma unroll
+      for (int d = 0; d < kSlices; ++d) acc[d] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int d = 0; d < kSlices; ++d) {
+#pragma unroll
+        for (int p = 0; p <= d; ++p) {
+          const int r = d - p;
+          acc[d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, as[p]),
+                                                           __builtin_bit_cast(v8bf, bs[r]), acc[d], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // the slices carry their 2^{-8(p+1)} scale, so acc[d] is already in units of
+        // the scaled operands; only the two row/column exponents remain
+        double sum = 0.0;
+#pragma unroll
+        for (int d = 0; d < kSlices; ++d) sum += (double)acc[d][e];
+        out[ct][e] += ldexp(sum, er[e] + eb);
       }
     }
-    // row 4q+e of the result needs that row's exponent (held by lane 4q+e)
+  }
+#pragma unroll
+  for (int ct = 0; ct < kMaxCT; ++ct) {
+    if (ct >= nct) break;
+    const int c = 16 * ct + rl;
+    if (c >= C) continue;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int er = __shfl(ea, 4 * q + e);
-      // the slices carry their 2^{-8(p+1)} scale, so acc[d] is already in units of
-      // the scaled operands; only the two row/column exponents remain
-      double s = 0.0;
-#pragma unroll
-      for (int d = 0; d < kSlices; ++d) s += (double)acc[d][e];
-      out[e] += ldexp(s, er + eb);
+      const int nn = n0 + 4 * q + e;
+      if (nn < nk) ft[((int64_t)(l * l + (c >> 1)) * nk + nn) * 2 + (c & 1)] = out[ct][e];
     }
   }
-  if (!vc) return;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int nn = n0 + 4 * q + e;
-    if (nn < nk) ft[((int64_t)(l * l + (c >> 1)) * nk + nn) * 2 + (c & 1)] = out[e];
+}
+
+__global__ __launch_bounds__(256) void radial_mfma_kernel(const int4* __restrict__ tiles, int ntiles,
+                                                          const double* __restrict__ W,
+                                                          const double* __restrict__ flm, double* __restrict__ ft,
+                                                          int Nr, int nk) {
+  radial_mfma_body(tiles, ntiles, W, flm, ft, Nr, nk, blockIdx.x);
+}
+
+// ---------------------------------------------------------------- many projectors
+// One launch per pass over the projectors of many grids: member q owns blocks
+// [start[q], start[q+1]) of the pass and runs the single-grid body on them, so
+// every member's arithmetic is that of its own launch.
+struct SpecMember {
+  InterpArgs ia;
+  const double* F;  // the projected field (system 0)
+  int64_t sys_stride;
+  const double* trig;
+  const double* leg;
+  const double* W;
+  double* cs;
+  double* flm;
+  double* ft;
+  const int2* tiles64;
+  const int4* tiles16;
+  int Nr, Nt, nk, lmax, mc, nsys, npair, ntiles64, ntiles16;
+};
+
+__device__ __forceinline__ int spec_member(const int* __restrict__ start, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
+  return lo;
+}
+
+template <int MC>
+__global__ __launch_bounds__(256) void phi_kernel_batch(const SpecMember* __restrict__ ms,
+                                                        const int* __restrict__ start, int n) {
+  const int q = spec_member(start, n, blockIdx.x);
+  const SpecMember& m = ms[q];
+  const int b = blockIdx.x - start[q];
+  const int gx = (m.Nr + 63) / 64, gy = (m.Nt + 3) / 4;
+  phi_body<MC>(m.ia, m.F, m.sys_stride, m.trig, m.cs, b % gx, (b / gx) % gy, b / (gx * gy));
+}
+
+__global__ __launch_bounds__(256) void theta_kernel_batch(const SpecMember* __restrict__ ms,
+                                                          const int* __restrict__ start, int n) {
+  const int q = spec_member(start, n, blockIdx.x);
+  const SpecMember& m = ms[q];
+  const int b = blockIdx.x - start[q];
+  const int gx = (m.Nr + 63) / 64;
+  theta_body(m.cs, m.leg, m.flm, m.Nr, m.Nt, m.mc, m.lmax, m.nsys, b % gx, b / gx);
+}
+
+template <int CPT>
+__global__ __launch_bounds__(256) void radial_f64_kernel_batch(const SpecMember* __restrict__ ms,
+                                                               const int* __restrict__ start, int n) {
+  const int q = spec_member(start, n, blockIdx.x);
+  const SpecMember& m = ms[q];
+  radial_f64_body<CPT>(m.tiles64, m.W, m.flm, m.ft, m.Nr, m.nk, blockIdx.x - start[q]);
+}
+
+__global__ __launch_bounds__(256) void radial_mfma_kernel_batch(const SpecMember* __restrict__ ms,
+                                                                const int* __restrict__ start, int n) {
+  const int q = spec_member(start, n, blockIdx.x);
+  const SpecMember& m = ms[q];
+  radial_mfma_body(m.tiles16, m.ntiles16, m.W, m.flm, m.ft, m.Nr, m.nk, blockIdx.x - start[q]);
 }
 
 // ---------------------------------------------------------------- detectors
@@ -555,6 +686,9 @@ struct kg_spec {
   int ntiles64 = 0;
   int4* d_tiles16 = nullptr;
   int ntiles16 = 0;
+  void* d_batch = nullptr;  // member descriptors of kg_spec_radial_batch (this projector leading)
+  size_t batch_bytes = 0;
+  hipEvent_t batch_done = nullptr;
   std::string err;
 };
 
@@ -590,10 +724,11 @@ kg_status dalloc(kg_spec* p, T** ptr, size_t count) {
 }
 
 void free_all(kg_spec* p) {
-  void* ptrs[] = {p->d_interp, p->d_trig, p->d_leg, p->d_kv,      p->d_r,       p->d_W,   p->d_cs,
-                  p->d_flm,    p->d_flmi, p->d_ft,  p->d_f,       p->d_tiles64, p->d_tiles16};
+  void* ptrs[] = {p->d_interp, p->d_trig, p->d_leg, p->d_kv, p->d_r,       p->d_W,       p->d_cs,
+                  p->d_flm,    p->d_flmi, p->d_ft,  p->d_f,  p->d_tiles64, p->d_tiles16, p->d_batch};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (p->batch_done) (void)hipEventDestroy(p->batch_done);
 }
 
 // Bracket search of domain_to_sph (FFT_sph.f90:555-611) along one axis:
@@ -777,10 +912,8 @@ kg_status build_radial(kg_spec* p, const HandleView& v) {
   std::vector<int2> t64;
   std::vector<int4> t16;
   for (int l = 0; l <= lmax; ++l) {
-    const int C = 2 * (2 * l + 1);
     for (int n0 = 0; n0 < nk; n0 += 64) t64.push_back(int2{l, n0});
-    for (int n0 = 0; n0 < nk; n0 += 16)
-      for (int c0 = 0; c0 < C; c0 += 16) t16.push_back(int4{l, n0, c0, 0});
+    for (int n0 = 0; n0 < nk; n0 += 16) t16.push_back(int4{l, n0, 0, 0});  // every column tile of l
   }
   p->ntiles64 = (int)t64.size();
   p->ntiles16 = (int)t16.size();
@@ -994,6 +1127,120 @@ kg_status kg_spec_radial(kg_spec* p, int32_t field, int32_t precision, double* f
   if (st == KG_OK) st = enqueue_radial(p, v.stream, precision);
   if (st != KG_OK) return st;
   return deliver(p, v.stream, p->d_ft, ft, (size_t)p->nlm * p->nk * 2, on_device);
+}
+
+kg_status kg_spec_radial_batch(kg_spec* const* ps, int32_t n, int32_t field, int32_t precision, double* const* fts,
+                               int32_t on_device) {
+  if (!ps || !fts || n < 1) return sfail(nullptr, KG_ERR_INVALID, "need n >= 1 projectors and outputs");
+  kg_spec* lead = ps[0];
+  if (!lead) return sfail(nullptr, KG_ERR_INVALID, "batch: null projector 0");
+  if (precision != KG_SPEC_FP64 && precision != KG_SPEC_SPLIT_BF16)
+    return sfail(lead, KG_ERR_INVALID, "unknown precision %d", precision);
+  HandleView lv;
+  kg_status st = view(lead, &lv);
+  if (st == KG_OK) st = check_field(lead, field);
+  if (st != KG_OK) return st;
+  std::vector<SpecMember> ms((size_t)n);
+  std::vector<int> start((size_t)4 * (n + 1), 0);  // [pass][member]: phi, theta, radial FP64, radial BF16
+  for (int q = 0; q < n; ++q) {
+    kg_spec* p = ps[q];
+    if (!p || !fts[q]) return sfail(lead, KG_ERR_INVALID, "batch: null projector or output %d", q);
+    if (p->device != lead->device) return sfail(lead, KG_ERR_INVALID, "batch: projectors on different devices");
+    if (p->mc != lead->mc)
+      return sfail(lead, KG_ERR_UNSUPPORTED, "batch: lmax <= 8 and lmax > 8 projectors cannot share a launch");
+    for (int o = 0; o < q; ++o)
+      if (ps[o] == p) return sfail(lead, KG_ERR_INVALID, "batch: projector %d repeated", q);
+    HandleView v;
+    if (handle_view(p->h, &v) != 0) return sfail(lead, KG_ERR_INVALID, "batch: projector %d has no handle", q);
+    // the member's pending work (its steps) completes before the batch reads its state
+    if (v.stream != lv.stream) SP_HIP(lead, hipStreamSynchronize(v.stream));
+    SpecMember& m = ms[(size_t)q];
+    m.ia = p->ia;
+    m.F = v.u[field];
+    m.sys_stride = v.G.sys_stride;
+    m.trig = p->d_trig;
+    m.leg = p->d_leg;
+    m.W = p->d_W;
+    m.cs = p->d_cs;
+    m.flm = p->d_flm;
+    m.ft = p->d_ft;
+    m.tiles64 = p->d_tiles64;
+    m.tiles16 = p->d_tiles16;
+    m.Nr = p->Nr;
+    m.Nt = p->Nt;
+    m.nk = p->nk;
+    m.lmax = p->lmax;
+    m.mc = p->mc;
+    m.nsys = p->nsys;
+    m.npair = p->npair;
+    m.ntiles64 = p->ntiles64;
+    m.ntiles16 = p->ntiles16;
+    const int gx = (p->Nr + 63) / 64;
+    const int64_t nb[4] = {(int64_t)gx * ((p->Nt + 3) / 4) * p->nsys, (int64_t)gx * ((p->npair + 3) / 4),
+                           (int64_t)p->ntiles64, (int64_t)(p->ntiles16 + 3) / 4};
+    for (int k = 0; k < 4; ++k) {
+      const int64_t next = (int64_t)start[(size_t)k * (n + 1) + q] + nb[k];
+      if (next > INT32_MAX) return sfail(lead, KG_ERR_INVALID, "batch: too many blocks");
+      start[(size_t)k * (n + 1) + q + 1] = (int)next;
+    }
+  }
+  const size_t mbytes = ((size_t)n * sizeof(SpecMember) + 255) & ~(size_t)255;
+  const size_t bytes = mbytes + start.size() * sizeof(int);
+  if (bytes > lead->batch_bytes) {
+    SP_HIP(lead, hipStreamSynchronize(lv.stream));  // the old table may still be read
+    if (lead->d_batch) (void)hipFree(lead->d_batch);
+    lead->d_batch = nullptr;
+    lead->batch_bytes = 0;
+    char* d = nullptr;
+    if (dalloc(lead, &d, bytes) != KG_OK) return KG_ERR_NOMEM;
+    lead->d_batch = d;
+    lead->batch_bytes = bytes;
+  }
+  if (!lead->batch_done) SP_HIP(lead, hipEventCreateWithFlags(&lead->batch_done, hipEventDisableTiming));
+  std::vector<char> host(bytes, 0);
+  std::memcpy(host.data(), ms.data(), (size_t)n * sizeof(SpecMember));
+  std::memcpy(host.data() + mbytes, start.data(), start.size() * sizeof(int));
+  // pageable source: the copy is staged before the call returns
+  SP_HIP(lead, hipMemcpyAsync(lead->d_batch, host.data(), bytes, hipMemcpyHostToDevice, lv.stream));
+  const SpecMember* dm = (const SpecMember*)lead->d_batch;
+  const int* ds = (const int*)((const char*)lead->d_batch + mbytes);
+  auto total = [&](int k) { return (unsigned)start[(size_t)k * (n + 1) + n]; };
+  if (total(0) > 0) {
+    if (lead->mc == 9)
+      hipLaunchKernelGGL(phi_kernel_batch<9>, dim3(total(0)), dim3(256), 0, lv.stream, dm, ds, n);
+    else
+      hipLaunchKernelGGL(phi_kernel_batch<17>, dim3(total(0)), dim3(256), 0, lv.stream, dm, ds, n);
+    SP_HIP(lead, hipGetLastError());
+  }
+  if (total(1) > 0) {
+    hipLaunchKernelGGL(theta_kernel_batch, dim3(total(1)), dim3(256), 0, lv.stream, dm, ds + (n + 1), n);
+    SP_HIP(lead, hipGetLastError());
+  }
+  if (precision == KG_SPEC_FP64 && total(2) > 0) {
+    if (lead->lmax <= 8)
+      hipLaunchKernelGGL(radial_f64_kernel_batch<9>, dim3(total(2)), dim3(256), 0, lv.stream, dm, ds + 2 * (n + 1), n);
+    else
+      hipLaunchKernelGGL(radial_f64_kernel_batch<17>, dim3(total(2)), dim3(256), 0, lv.stream, dm, ds + 2 * (n + 1),
+                         n);
+    SP_HIP(lead, hipGetLastError());
+  } else if (precision == KG_SPEC_SPLIT_BF16 && total(3) > 0) {
+    hipLaunchKernelGGL(radial_mfma_kernel_batch, dim3(total(3)), dim3(256), 0, lv.stream, dm, ds + 3 * (n + 1), n);
+    SP_HIP(lead, hipGetLastError());
+  }
+  for (int q = 0; q < n; ++q) {
+    kg_spec* p = ps[q];
+    SP_HIP(lead, hipMemcpyAsync(fts[q], p->d_ft, (size_t)p->nlm * p->nk * 2 * 8,
+                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, lv.stream));
+  }
+  // later work on any member (its steps, its own projections) waits for the batch
+  SP_HIP(lead, hipEventRecord(lead->batch_done, lv.stream));
+  for (int q = 1; q < n; ++q) {
+    HandleView v;
+    if (handle_view(ps[q]->h, &v) == 0 && v.stream != lv.stream)
+      SP_HIP(lead, hipStreamWaitEvent(v.stream, lead->batch_done, 0));
+  }
+  if (!on_device) SP_HIP(lead, hipStreamSynchronize(lv.stream));
+  return KG_OK;
 }
 
 kg_status kg_spec_radial_flm(kg_spec* p, const double* flm, int32_t precision, double* ft) {

@@ -114,3 +114,27 @@ class SpectralProjector:
         out = np.zeros((self.nlm, self.nk, 2))
         self._check(self._lib.kg_spec_radial_flm(self._p, _ptr(src), _PRECISIONS[precision], _ptr(out)), self._p)
         return out[..., 0] + 1j * out[..., 1]
+
+
+def radial_batch(projectors, field=0, precision="split_bf16", out_device_ptrs=None):
+    """``radial`` of many projectors (grids of any size, all lmax <= 8 or all
+    > 8) with one launch per pass over all of them (``kg_spec_radial_batch``):
+    a list of complex [lm][n] arrays, equal bit for bit to each projector's
+    own ``radial``.  With out_device_ptrs (one device address of nlm*nk*2
+    doubles per projector) the results stay on the GPU and the call returns
+    None without waiting."""
+    projectors = list(projectors)
+    n = len(projectors)
+    if n == 0:
+        return []
+    lib = projectors[0]._lib
+    ps = (C.c_void_p * n)(*[p._p for p in projectors])
+    prec = _PRECISIONS[precision]
+    if out_device_ptrs is not None:
+        outs = (C.c_void_p * n)(*[C.c_void_p(int(a)) for a in out_device_ptrs])
+        projectors[0]._check(lib.kg_spec_radial_batch(ps, n, field, prec, outs, 1), projectors[0]._p)
+        return None
+    arrays = [np.zeros((p.nlm, p.nk, 2)) for p in projectors]
+    outs = (C.c_void_p * n)(*[a.ctypes.data for a in arrays])
+    projectors[0]._check(lib.kg_spec_radial_batch(ps, n, field, prec, outs, 0), projectors[0]._p)
+    return [a[..., 0] + 1j * a[..., 1] for a in arrays]

@@ -30,6 +30,8 @@ CASES = [
                            bc="inflow")),
     ("wide_r_tiles", dict(Nx=140, Ny=11, Nz=9, xmin=1.8, xmax=30.0, metric="Kerr_Schild", a=0.3, M_bh=1.0, m=0.2,
                           bc="outflow")),
+    ("tall_theta", dict(Nx=70, Ny=37, Nz=12, xmin=1.8, xmax=15.0, metric="Kerr_Schild", a=0.5, M_bh=1.0, m=0.2,
+                        bc="sommerfeld")),
 ]
 
 
@@ -74,16 +76,19 @@ def test_fd4_steps_match_oracle(scheme, name, p):
     gpu.close()
 
 
-@pytest.mark.parametrize("kc", [1, 2, 3, 64])
-def test_fd4_launch_shape_does_not_change_results(kc):
-    p = CASES[1][1]
+@pytest.mark.parametrize("kc,tj", [(1, 4), (2, 8), (3, 8), (64, 8)])
+@pytest.mark.parametrize("case", [1, 6])
+def test_fd4_launch_shape_does_not_change_results(kc, tj, case):
+    """The fd4 stages run tiles of 8 theta rows whatever tj; the phi chunk
+    varies (1, 2, 3 planes and a whole column)."""
+    p = CASES[case][1]
     sim, gpu, u0 = _pair(p, "rk4", seed=5)
     sim.u[...] = u0
     gpu.set_u_soa(u0[None])
-    gpu.set_tuning(kc, 4)  # the fd4 stages choose their own theta rows per workgroup
+    gpu.set_tuning(kc, tj)
     sim.step("rk4", 4)
     gpu.step(4)
-    assert np.array_equal(gpu.u_soa()[0], sim.u), kc
+    assert np.array_equal(gpu.u_soa()[0], sim.u), (kc, tj)
     gpu.close()
 
 

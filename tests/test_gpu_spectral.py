@@ -176,3 +176,59 @@ def test_radial_contraction_alone_at_large_nr():
     err = _rel(sp.radial_from_flm(flm, "split_bf16"), ft_o)
     print("Nr=2048 split-bf16 error: %.3e" % err, flush=True)
     assert err < BF16_TOL
+
+
+def test_radial_batch_equals_each_projector_alone():
+    """kg_spec_radial_batch over uneven grids (sizes, real and complex states,
+    both bases, different wave-number counts): every member's spectrum equals
+    its own kg_spec_radial bit for bit, in FP64 and split BF16, host and
+    device outputs, and after the members step again."""
+    import torch
+
+    from kg_amd.spectral import radial_batch
+
+    shapes = [(40, 9, 17, False, "zeros", 24), (64, 13, 9, True, "uniform", 40), (33, 7, 12, False, "uniform", 16),
+              (96, 17, 33, True, "zeros", 64), (256, 17, 33, False, "uniform", 512)]
+    lms, sps = [], []
+    for q, (Nx, Ny, Nz, cplx, basis, nk) in enumerate(shapes):
+        lm = _make(Nx, Ny, Nz, 1.8, 1.8 + 0.25 * Nx, 40 + q, is_complex=cplx)
+        lms.append(lm)
+        sps.append(SpectralProjector(lm, lmax=8, nk=nk, basis=basis, kmax=3.0))
+    for rnd in range(2):
+        for prec in ("fp64", "split_bf16"):
+            for field in (0, 4):
+                print("spectral batch", rnd, prec, field, flush=True)
+                got = radial_batch(sps, field, prec)
+                for q, sp in enumerate(sps):
+                    want = sp.radial(field, prec)
+                    assert np.array_equal(got[q], want), (rnd, prec, field, q)
+        outs = [torch.zeros((sp.nlm, sp.nk, 2), dtype=torch.float64, device="cuda") for sp in sps]
+        radial_batch(sps, 0, "split_bf16", out_device_ptrs=[o.data_ptr() for o in outs])
+        lms[0].sync()
+        torch.cuda.synchronize()
+        for q, sp in enumerate(sps):
+            want = sp.radial(0, "split_bf16")
+            got = outs[q].cpu().numpy()
+            assert np.array_equal(got[..., 0] + 1j * got[..., 1], want), ("device", q)
+        for lm in lms:
+            lm.step(3)
+    for sp in sps:
+        sp.close()
+    for lm in lms:
+        lm.close()
+
+
+def test_radial_batch_refuses_mixed_lmax_classes():
+    from kg_amd import KGError
+    from kg_amd.spectral import radial_batch
+
+    a = _make(24, 7, 9, 1.8, 8.0, 1)
+    b = _make(24, 7, 9, 1.8, 8.0, 2)
+    sa = SpectralProjector(a, lmax=8, nk=8)
+    sb = SpectralProjector(b, lmax=10, nk=8)
+    with pytest.raises(KGError):
+        radial_batch([sa, sb], 0, "fp64")
+    with pytest.raises(KGError):
+        radial_batch([sa, sa], 0, "fp64")
+    for x in (sa, sb, a, b):
+        x.close()
