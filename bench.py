@@ -22,10 +22,12 @@ spherical-Bessel projection (l <= 8, 512 k, split BF16 and FP64), the serving
 loop (Poisson requests, continuous batching, p50/p99 time-to-result), and
 the reference's random_shell initial data built on the GPU for the full grid,
 the reference's own production step (PIRK2 with its 2nd-order stencils) on the
-full grid, and the CPU oracle on a bounded sample.
+full grid, the PIRK/RK3 workload with its 2:1 radial patch, and the CPU oracle
+on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--operators fd4|ref2] [--other-steps K]
-                [--pirk2-steps K] [--batched N] [--serve N] [--serve-small N] [--ic N] [--ic-only]
+                [--pirk2-steps K] [--refined-steps K] [--batched N] [--serve N] [--serve-small N] [--ic N]
+                [--ic-only]
 """
 from __future__ import annotations
 
@@ -311,6 +313,68 @@ KERNELS = {"ref2": "stage_kernel (RHS + RK4 stage combine, the reference's 2nd-o
            "fd4": "stage4_kernel (RHS + RK4 stage combine, the 4th-order stencil set)"}
 
 
+def refined_workload(scheme, steps, warmup=5):
+    """The PIRK/RK3 workload: base 1024 x 64 x 128 on r in [1.8, 100], m = 0.3,
+    the l = 1, m = 1 pulse at r0 = 8 (width 1.5), the reference's 2nd-order
+    stencils, with its 2:1 patch on [1.8, 10] (two fine steps per base step,
+    buffer refill and injection), and the same base grid alone.  Updates
+    count every level: base points per base step plus fine points per fine
+    step."""
+    import torch
+    import kg_amd
+    from kg_amd.initial_data import shell_state_device
+    from kg_amd.refinement import RefinedLinesMethod
+
+    Nx, Ny, Nz, xmin, xmax = 1024, 65, 129, 1.8, 100.0
+    dx0 = ((xmin + 1.0 * (xmax - xmin)) - xmin) / Nx
+    kw = dict(Nx=Nx, Ny=Ny, Nz=Nz, xmin=xmin, xmax=xmax, CFL=0.25, m=0.3, boundary_type_rmax="outflow",
+              scheme=scheme)
+
+    def init(lv, lxmax):
+        u0 = shell_state_device(lv.Nx, lv.Ny, lv.Nz, 2, xmin, lxmax, 8.0, 1.5, 1e-3, {(1, 1): 1.0})
+        torch.cuda.synchronize()
+        lv.copy_state_device(u0.data_ptr(), to_handle=True)
+        lv.boundary()
+        lv.set_tuning(8, 8)
+
+    print("bench: refined %s workload" % scheme, file=sys.stderr, flush=True)
+    gpu = RefinedLinesMethod(10.0, **kw)
+    init(gpu.base, xmin + gpu.base.Nx * dx0)
+    init(gpu.fine, xmin + gpu.fine.Nx * dx0 / 2.0)
+    gpu.step(warmup)
+    gpu.base.sync()
+    gpu.fine.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu.step(steps)
+    gpu.base.sync()
+    gpu.fine.sync()
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    base_pts = Nx * (Ny - 1) * (Nz - 1)
+    fine_pts = (gpu.fine.Nx - 4) * (Ny - 1) * (Nz - 1)  # physical fine columns, buffer included
+    fine_nx = gpu.fine.Nx
+    gpu.close()
+    alone = kg_amd.LinesMethod(**kw)
+    init(alone, xmax)
+    alone.step(warmup)
+    alone.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    alone.step(steps)
+    alone.sync()
+    torch.cuda.synchronize()
+    t_alone = time.perf_counter() - t0
+    alone.close()
+    return {"scheme": scheme, "steps": steps, "base_grid": [Nx, Ny - 1, Nz - 1], "fine_nr": fine_nx,
+            "refined_ms_per_base_step": t_ref / steps * 1e3,
+            "refined_updates_per_s": (base_pts + 2 * fine_pts) * steps / t_ref,
+            "unrefined_ms_per_step": t_alone / steps * 1e3,
+            "unrefined_updates_per_s": base_pts * steps / t_alone,
+            "workload": "base 1024 x 64 x 128, r in [1.8, 100], m=0.3, l=1 m=1 pulse r0=8 w=1.5, 2:1 patch on "
+                        "[1.8, 10], reference's 2nd-order stencils, outflow"}
+
+
 def plain(operators, steps, warmup, kc=0, tj=8, scheme=None):
     """The plain single-grid workload with one operator set (and the workload's
     RK4 unless `scheme` names another integrator): state resident in HBM,
@@ -421,6 +485,8 @@ def main():
     ap.add_argument("--batched", type=int, default=256,
                     help="members of the uneven batched workload to time beside the plain path (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--refined-steps", type=int, default=200,
+                    help="base steps of the PIRK/RK3 workload with its 2:1 patch (PIRK2, PIRK3, RK3); 0 skips it")
     ap.add_argument("--ic", type=int, default=512,
                     help="Cartesian points per axis of the random_shell initial data to time (0: skip)")
     ap.add_argument("--ic-only", action="store_true", help="time only the initial data and exit")
@@ -481,6 +547,8 @@ def main():
     if args.pirk2_steps > 0:
         print("bench: the reference's PIRK2 + 2nd-order step", file=sys.stderr, flush=True)
         out["plain_ref2_pirk2"] = plain("ref2", args.pirk2_steps, 5, scheme="pirk2")
+    if args.refined_steps > 0:
+        out["refined_pirk"] = [refined_workload(sch, args.refined_steps) for sch in ("pirk2", "pirk3", "rk3")]
     if args.ic > 0:
         print("bench: initial data, ncart %d" % args.ic, file=sys.stderr, flush=True)
         out["initial_data"] = initial_data(args.ic)

@@ -90,3 +90,45 @@ def test_pirk_workload_refined_1000_base_steps(scheme, operators):
         print("  250 base steps more", flush=True)
     _check(osims, gpu, scheme)
     gpu.close()
+
+
+@pytest.mark.parametrize("scheme", ["pirk2", "pirk3", "rk3"])
+def test_pirk_workload_full_size_refined_is_shape_invariant(scheme):
+    """The PIRK/RK3 workload at its full size: base 1024 x 64 x 128 on
+    r in [1.8, 100] (m = 0.3, the l = 1, m = 1 pulse at r0 = 8, width 1.5)
+    with its 2:1 patch on [1.8, 10]: 100 base steps (200 fine) under two
+    launch shapes give the same two levels bit for bit, finite and nonzero."""
+    import torch
+
+    from kg_amd.initial_data import shell_state_device
+
+    Nx, Ny, Nz, xmin, xmax = 1024, 65, 129, 1.8, 100.0
+    dx0 = ((xmin + 1.0 * (xmax - xmin)) - xmin) / Nx
+    results = []
+    for kc, tj in ((8, 8), (32, 4)):
+        print("full-size refined", scheme, "kc", kc, "tj", tj, flush=True)
+        gpu = RefinedLinesMethod(10.0, Nx=Nx, Ny=Ny, Nz=Nz, xmin=xmin, xmax=xmax, CFL=0.25, m=0.3,
+                                 boundary_type_rmax="outflow", scheme=scheme)
+        for lv in (gpu.base, gpu.fine):
+            lv_xmax = xmin + lv.Nx * (dx0 if lv is gpu.base else dx0 / 2.0)
+            u0 = shell_state_device(lv.Nx, lv.Ny, lv.Nz, 2, xmin, lv_xmax, 8.0, 1.5, 1e-3, {(1, 1): 1.0})
+            torch.cuda.synchronize()
+            lv.copy_state_device(u0.data_ptr(), to_handle=True)
+            lv.boundary()
+            lv.set_tuning(kc, tj)
+        gpu.step(40)
+        gpu.step(60)
+        out = []
+        for lv in (gpu.base, gpu.fine):
+            buf = torch.empty((5, lv.nk, lv.nj, lv.ni), dtype=torch.float64, device="cuda")
+            lv.sync()
+            lv.copy_state_device(buf.data_ptr(), to_handle=False)
+            torch.cuda.synchronize()
+            out.append(buf)
+        gpu.close()
+        results.append(out)
+    for lvl in range(2):
+        a, b = results[0][lvl], results[1][lvl]
+        assert bool(torch.isfinite(a).all())
+        assert float(a[0].abs().max()) > 1e-6
+        assert torch.equal(a, b), ("level", lvl)
