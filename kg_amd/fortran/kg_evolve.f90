@@ -8,19 +8,23 @@
 !   data of main/initial_conditions.f90 built by the engine
 !   optional further arguments: batch=<K>: K copies of the grid advanced
 !   together in batched launches (the driver stops with an error unless all K
-!   agree); bc=<outflow|inflow|sommerfeld>: the outer boundary (outflow)
+!   agree); bc=<outflow|inflow|sommerfeld>: the outer boundary (outflow);
+!   refine=<r_patch>,<fine_in>,<fine_out>: a 2:1 radial patch on
+!   [xmin, r_patch] (buffer 16): fine_in/fine_out hold the fine level's u
+!   (same layout, its own Nx), in.bin/out.bin the base level's
 program kg_evolve
   use kg_hip_mod
   implicit none
   type(kg_lines_method) :: MoL
   type(kg_lines_method), allocatable :: batch(:)
-  complex(kind=8), allocatable :: u(:,:,:,:), v(:,:,:,:)
-  character(len=256) :: arg, fin, fout, scheme
+  type(kg_refined_lines_method) :: pair
+  complex(kind=8), allocatable :: u(:,:,:,:), v(:,:,:,:), uf(:,:,:,:)
+  character(len=256) :: arg, fin, fout, scheme, ffin, ffout
   character(len=16) :: bc
-  integer :: ia
+  integer :: ia, ic
   integer :: Nx, Ny, Nz, nsteps, l, g, isch, ncart, ops, nb, q
   integer(kind=8) :: seed
-  real(kind=8) :: xmin, xmax, m, mu, sigma, amp
+  real(kind=8) :: xmin, xmax, m, mu, sigma, amp, r_patch
   character(len=64) :: ictype
 
   g = 2
@@ -36,10 +40,20 @@ program kg_evolve
   call get_command_argument(10, fout)
   nb = 1
   bc = 'outflow'
+  r_patch = -1.0d0
   do ia = 11, command_argument_count()
     call get_command_argument(ia, arg)
     if (arg(1:6) == 'batch=') read(arg(7:), *) nb
     if (arg(1:3) == 'bc=') bc = trim(arg(4:))
+    if (arg(1:7) == 'refine=') then
+      arg = arg(8:)
+      ic = index(arg, ',')
+      read(arg(1:ic-1), *) r_patch
+      arg = arg(ic+1:)
+      ic = index(arg, ',')
+      ffin = arg(1:ic-1)
+      ffout = arg(ic+1:)
+    end if
   end do
 
   ops = KG_OPS_REF2
@@ -52,9 +66,37 @@ program kg_evolve
   case default;   isch = KG_SCHEME_RK4
   end select
 
+  allocate(u(5, -g:Nx+g, -g:Ny+g, 0:Nz))
+  if (r_patch > 0.0d0) then
+    ! the base grid with its refined patch, Berger-Oliger sub-stepping
+    pair = kg_refined_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
+                                               trim(bc), 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., &
+                                               r_patch, 16, ops)
+    allocate(uf(5, -g:pair%fine%Nx+g, -g:Ny+g, 0:Nz))
+    open(unit=10, file=trim(fin), access='stream', form='unformatted', status='old')
+    read(10) u
+    close(10)
+    open(unit=10, file=trim(ffin), access='stream', form='unformatted', status='old')
+    read(10) uf
+    close(10)
+    call pair%base%upload(u)
+    call pair%fine%upload(uf)
+    call pair%step(nsteps)
+    call pair%base%download(u)
+    call pair%fine%download(uf)
+    open(unit=11, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(11) u
+    close(11)
+    open(unit=11, file=trim(ffout), access='stream', form='unformatted', status='replace')
+    write(11) uf
+    close(11)
+    print '(A,I8,A,I6,A,ES24.16,A,ES24.16)', 'refined steps ', nsteps, ' fine Nx ', pair%fine%Nx, ' t ', &
+          pair%base%t, ' dt ', pair%base%dt
+    call pair%destroy()
+    stop
+  end if
   MoL = kg_lines_method_constructor(5, g, xmin, xmax, Nx, Ny, Nz, 0.0d0, 1.0d0, 0.25d0, &
                                     trim(bc), 'Kerr_Schild', 0.0d0, 1.0d0, m, isch, .true., ops)
-  allocate(u(5, -g:Nx+g, -g:Ny+g, 0:Nz))
   if (fin(1:3) == 'ic=') then
     read(fin(4:), *) ictype, mu, sigma, ncart, seed
     call MoL%initial_conditions(trim(ictype), mu, sigma, ncart, seed, amp)

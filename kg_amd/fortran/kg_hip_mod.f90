@@ -39,6 +39,12 @@ module kg_hip_mod
     type(c_ptr) :: uniforms
   end type kg_ic_desc
 
+  ! two refinement levels (include/kg.h: struct kg_refine_desc)
+  type, bind(C), public :: kg_refine_desc
+    real(c_double) :: r_patch
+    integer(c_int32_t) :: buffer, reserved
+  end type kg_refine_desc
+
   interface
     function kg_create(desc, h) bind(C, name="kg_create") result(st)
       import :: kg_desc, c_ptr, c_int32_t
@@ -141,6 +147,41 @@ module kg_hip_mod
       real(c_double), intent(out) :: amp
       integer(c_int32_t) :: st
     end function kg_initial_data
+    function kg_get_dims(h, dims) bind(C, name="kg_get_dims") result(st)
+      import :: c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: dims(8)
+      integer(c_int32_t) :: st
+    end function kg_get_dims
+    function kg_refined_create(base, desc, r) bind(C, name="kg_refined_create") result(st)
+      import :: kg_desc, kg_refine_desc, c_ptr, c_int32_t
+      type(kg_desc), intent(in) :: base
+      type(kg_refine_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: r
+      integer(c_int32_t) :: st
+    end function kg_refined_create
+    function kg_refined_levels(r, hb, hf) bind(C, name="kg_refined_levels") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: r
+      type(c_ptr), intent(out) :: hb, hf
+      integer(c_int32_t) :: st
+    end function kg_refined_levels
+    function kg_refined_step(r, nsteps) bind(C, name="kg_refined_step") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: r
+      integer(c_int32_t), value :: nsteps
+      integer(c_int32_t) :: st
+    end function kg_refined_step
+    function kg_refined_destroy(r) bind(C, name="kg_refined_destroy") result(st)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: r
+      integer(c_int32_t) :: st
+    end function kg_refined_destroy
+    function kg_refined_last_error(r) bind(C, name="kg_refined_last_error") result(msg)
+      import :: c_ptr
+      type(c_ptr), value :: r
+      type(c_ptr) :: msg
+    end function kg_refined_last_error
     function kg_ic_last_error() bind(C, name="kg_ic_last_error") result(msg)
       import :: c_ptr
       type(c_ptr) :: msg
@@ -173,6 +214,19 @@ module kg_hip_mod
   end type kg_lines_method
 
   public :: kg_lines_method_constructor
+
+  ! Two fixed refinement levels (no counterpart in the reference): `base` and
+  ! `fine` are views of the two levels for upload/download and diagnostics;
+  ! `step(n)` advances both (n base steps, two fine steps each).
+  type, public :: kg_refined_lines_method
+    type(c_ptr) :: r = c_null_ptr
+    type(kg_lines_method) :: base, fine
+  contains
+    procedure :: step => krl_step
+    procedure :: destroy => krl_destroy
+  end type kg_refined_lines_method
+
+  public :: kg_refined_lines_method_constructor
 
 contains
 
@@ -208,7 +262,23 @@ contains
     integer, intent(in), optional :: op_set
     type(kg_lines_method) :: this
     type(kg_desc) :: d
-    real(c_double) :: t, dt
+    type(c_ptr) :: h
+    d = make_desc(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, CFL, boundary_type_rmax, metric, a, M_bh, m, &
+                  scheme, is_complex, op_set)
+    call kg_check(kg_create(d, h), c_null_ptr)
+    call view_init(this, h, tmin, tmax, CFL)
+  end function kg_lines_method_constructor
+
+  ! the engine's descriptor from lines_method_constructor's arguments
+  function make_desc(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, CFL, boundary_type_rmax, metric, a, M_bh, m, &
+                     scheme, is_complex, op_set) result(d)
+    integer, intent(in) :: nvars, g_pts, Nx, Ny, Nz
+    real(kind=8), intent(in) :: xmin, xmax, tmin, CFL, a, M_bh, m
+    character(len=*), intent(in) :: boundary_type_rmax, metric
+    integer, intent(in) :: scheme
+    logical, intent(in) :: is_complex
+    integer, intent(in), optional :: op_set
+    type(kg_desc) :: d
     d%nvars = nvars; d%g_pts = g_pts; d%Nx = Nx; d%Ny = Ny; d%Nz = Nz
     d%scheme = scheme
     if (metric == 'Kerr_Schild') then
@@ -243,15 +313,85 @@ contains
     if (present(op_set)) d%op_set = int(op_set, c_int32_t)
     d%xmin = xmin; d%xmax = xmax; d%tmin = tmin; d%CFL = CFL; d%dt = 0.0d0
     d%a = a; d%M_bh = M_bh; d%m = m
-    call kg_check(kg_create(d, this%h), c_null_ptr)
-    this%nvars = nvars; this%g_pts = g_pts; this%Nx = Nx; this%Ny = Ny; this%Nz = Nz
-    this%iL = 0; this%iR = Nx; this%jL = 0; this%jR = Ny; this%kL = 0; this%kR = Nz
-    call kg_check(kg_get_time(this%h, t, dt), this%h)
+  end function make_desc
+
+  ! the Fortran view of handle h: sizes and index bounds from the engine
+  subroutine view_init(this, h, tmin, tmax, CFL)
+    type(kg_lines_method), intent(inout) :: this
+    type(c_ptr), intent(in) :: h
+    real(kind=8), intent(in) :: tmin, tmax, CFL
+    integer(c_int64_t) :: dims(8)
+    real(c_double) :: t, dt
+    this%h = h
+    call kg_check(kg_get_dims(h, dims), h)
+    this%nvars = 5; this%Nx = int(dims(1)); this%Ny = int(dims(2)); this%Nz = int(dims(3))
+    this%g_pts = int(dims(4))
+    this%iL = 0; this%iR = this%Nx; this%jL = 0; this%jR = this%Ny; this%kL = 0; this%kR = this%Nz
+    call kg_check(kg_get_time(h, t, dt), h)
     this%t = t; this%dt = dt
     this%tmin = tmin; this%tmax = tmax; this%CFL = CFL
     this%Nt = int((tmax - tmin) / dt)           ! lines_method_lib.f90:115-116
     if (mod(this%Nt, 2) /= 0) this%Nt = this%Nt + 1
-  end function kg_lines_method_constructor
+  end subroutine view_init
+
+  ! The base grid of lines_method_constructor's arguments with a 2:1 radial
+  ! patch on [xmin, r_patch] and `buffer` fine columns beyond it (16 if 0).
+  function kg_refined_lines_method_constructor(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, tmax, CFL, &
+                                               boundary_type_rmax, metric, a, M_bh, m, scheme, is_complex, &
+                                               r_patch, buffer, op_set) result(this)
+    integer, intent(in) :: nvars, g_pts, Nx, Ny, Nz
+    real(kind=8), intent(in) :: xmin, xmax, tmin, tmax, CFL, a, M_bh, m, r_patch
+    character(len=*), intent(in) :: boundary_type_rmax, metric
+    integer, intent(in) :: scheme, buffer
+    logical, intent(in) :: is_complex
+    integer, intent(in), optional :: op_set
+    type(kg_refined_lines_method) :: this
+    type(kg_desc) :: d
+    type(kg_refine_desc) :: rd
+    type(c_ptr) :: hb, hf
+    d = make_desc(nvars, g_pts, xmin, xmax, Nx, Ny, Nz, tmin, CFL, boundary_type_rmax, metric, a, M_bh, m, &
+                  scheme, is_complex, op_set)
+    rd%r_patch = r_patch
+    rd%buffer = int(buffer, c_int32_t)
+    rd%reserved = 0
+    if (kg_refined_create(d, rd, this%r) /= KG_OK) call refined_fail(this%r)
+    if (kg_refined_levels(this%r, hb, hf) /= KG_OK) call refined_fail(this%r)
+    call view_init(this%base, hb, tmin, tmax, CFL)
+    call view_init(this%fine, hf, tmin, tmax, CFL)
+  end function kg_refined_lines_method_constructor
+
+  subroutine refined_fail(r)
+    type(c_ptr), intent(in) :: r
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    call c_f_pointer(kg_refined_last_error(r), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    print*, 'kg engine error (refinement): ', transfer(msg(1:n), repeat(' ', n))
+    stop
+  end subroutine refined_fail
+
+  subroutine krl_step(this, nsteps)
+    class(kg_refined_lines_method), intent(inout) :: this
+    integer, intent(in) :: nsteps
+    if (kg_refined_step(this%r, int(nsteps, c_int32_t)) /= KG_OK) call refined_fail(this%r)
+    call sync_time(this%base)
+    call sync_time(this%fine)
+  end subroutine krl_step
+
+  ! frees both levels with the pair
+  subroutine krl_destroy(this)
+    class(kg_refined_lines_method), intent(inout) :: this
+    integer(c_int32_t) :: st
+    if (.not. c_associated(this%r)) return
+    st = kg_refined_destroy(this%r)
+    this%r = c_null_ptr
+    this%base%h = c_null_ptr
+    this%fine%h = c_null_ptr
+  end subroutine krl_destroy
 
   subroutine klm_upload(this, u)
     class(kg_lines_method), intent(inout) :: this

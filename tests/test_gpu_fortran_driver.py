@@ -84,3 +84,50 @@ def test_fortran_driver_initial_data(tmp_path, kind, nsteps):
     for f in range(5):
         scale = max(np.abs(want[f]).max(), 1e-300)
         assert np.abs(got[f] - want[f]).max() <= 1e-12 * scale, (kind, f)
+
+
+@pytest.mark.parametrize("scheme,bc", [("pirk2", "outflow"), ("rk3_fd4", "sommerfeld")])
+def test_fortran_driver_refined_matches_oracle(tmp_path, scheme, bc):
+    """The two-level refinement through the Fortran binding
+    (kg_refined_lines_method): base and fine levels read from and written to
+    the reference's u(5, i, j, k) layout, both equal to the oracle's
+    refinement (oracle/refinement.py) bit for bit after 8 base steps."""
+    from oracle.refinement import RefinedSim
+
+    Nx, Ny, Nz, xmin, xmax, m, r_patch = 40, 9, 13, 1.8, 12.0, 0.2, 5.0
+    order = 4 if scheme.endswith("_fd4") else 2
+    sims = [RefinedSim(Nx, Ny, Nz, 2, xmin, xmax, "Kerr_Schild", 0.0, 1.0, m, bc, 0.25, r_patch, order=order)
+            for _ in range(2)]
+    files = {}
+    for lv_name in ("base", "fine"):
+        parts = []
+        for q, R in enumerate(sims):
+            lv = getattr(R, lv_name)
+            u0 = shell_state(lv.Nx, lv.Ny, lv.Nz, 2, xmin, xmin + lv.Nx * lv.dx, 4.5 + q, 1.2, 1.0 - 0.6 * q,
+                             {(1, 1): 1.0, (2, 0): 0.3})
+            if order == 4:
+                u0 = lv.fill(u0.copy())
+            lv.u[...] = u0
+            parts.append(u0)
+        u = parts[0] + 1j * parts[1]
+        uf = np.asfortranarray(np.transpose(u, (0, 3, 2, 1)))
+        fin, fout = tmp_path / (lv_name + "_in.bin"), tmp_path / (lv_name + "_out.bin")
+        uf.ravel(order="F").tofile(fin)
+        files[lv_name] = (fin, fout, uf.shape)
+    nsteps = 8
+    print("fortran driver refined", scheme, bc, flush=True)
+    extra = ["refine=%r,%s,%s" % (r_patch, files["fine"][0], files["fine"][1])] + (["bc=" + bc] if bc != "outflow"
+                                                                                    else [])
+    r = subprocess.run([EXE, str(Nx), str(Ny), str(Nz), repr(xmin), repr(xmax), repr(m), scheme, str(nsteps),
+                        str(files["base"][0]), str(files["base"][1])] + extra, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "refined steps" in r.stdout, r.stdout
+    for R in sims:
+        R.step(scheme.replace("_fd4", ""), nsteps)
+    for lv_name in ("base", "fine"):
+        fin, fout, shape = files[lv_name]
+        got = np.fromfile(fout, dtype=np.complex128).reshape(shape, order="F")
+        got = np.transpose(got, (0, 3, 2, 1))
+        assert np.array_equal(got.real, getattr(sims[0], lv_name).u), (lv_name, "re")
+        assert np.array_equal(got.imag, getattr(sims[1], lv_name).u), (lv_name, "im")
