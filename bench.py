@@ -145,6 +145,18 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     batch_step(members, steps)
     members[0].sync()
     tb = time.perf_counter() - t0
+    # the same batch as persistent launches (wg workgroups per CU march all
+    # tiles of a stage in turn), after a warm-up step that builds its plan
+    persistent = {}
+    for wg in (1, 2):
+        members[0].set_batch_persistent(wg)
+        batch_step(members, 1)
+        members[0].sync()
+        t0 = time.perf_counter()
+        batch_step(members, steps)
+        members[0].sync()
+        persistent[str(wg)] = pts * steps / (time.perf_counter() - t0)
+    members[0].set_batch_persistent(0)
     # the plain path at its own best shape for each grid (its autotuner, outside
     # the timed region), then a warm-up step (graph capture)
     picks = {}
@@ -166,6 +178,7 @@ def batched_vs_plain(n, steps, tuning, operators, specs=None, spectral=True):
     return {"members": n, "steps": steps, "operators": operators, "batch_kc": batch_kc,
             "plain_tuning": "autotuned per grid", "plain_picks_kc_tj": dict(sorted(picks.items())),
             "grid_points": pts, "batched_updates_per_s": vb,
+            "batched_persistent_updates_per_s": persistent,
             "plain_updates_per_s": vp, "speedup": vb / vp,
             "workload": "uneven RK4 batch: Nr~U{256..2048}, (Ntheta,Nphi) in {(16,32),(32,64),(64,128)}, "
                         "m~U[0,0.5], M~U[0.5,2], seed 20261203",

@@ -178,6 +178,12 @@ kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
  * it: calls on one GPU that overlap (from other threads, or on other
  * streams than an earlier call's still-running lead stream) race. */
 kg_status kg_batch_step_noalloc(kg_handle* const* hs, int32_t n, int32_t nsteps);
+/* Batched stage launches led by handle h (it is hs[0]): wg_per_cu = 0 (the
+ * default) launches one workgroup per tile; wg_per_cu > 0 makes each stage
+ * one persistent launch of wg_per_cu workgroups per compute unit that march
+ * the tiles of all members in turn.  Results do not depend on it.  No
+ * reference counterpart. */
+kg_status kg_set_batch_persistent(kg_handle* h, int32_t wg_per_cu);
 /* waits for the handle's stream */
 kg_status kg_sync(kg_handle* h);
 

@@ -86,6 +86,7 @@ SIGNATURES = {
     "kg_sync": (C.c_int32, [_vp]),
     "kg_batch_step": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int32]),
     "kg_batch_step_noalloc": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int32]),
+    "kg_set_batch_persistent": (C.c_int32, [_vp, C.c_int32]),
     "kg_set_tuning": (C.c_int32, [_vp, C.c_int32, C.c_int32]),
     "kg_profile_enable": (C.c_int32, [_vp, C.c_int32]),
     "kg_profile_read": (C.c_int32, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),

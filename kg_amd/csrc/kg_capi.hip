@@ -64,6 +64,7 @@ struct BatchPlan {
   char* dev = nullptr;
   uint64_t gen = 0;
   hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int nwg = 0;  // workgroups of a persistent stage launch (0: one per tile)
   BatchPlan() = default;
   BatchPlan(BatchPlan&&) = default;
   BatchPlan& operator=(BatchPlan&&) = default;
@@ -93,6 +94,8 @@ struct kg_handle {
   int px = 0;           // RK4: which of the two state buffers holds u
   int tune_kc = 32;     // phi planes per workgroup of the stage kernel
   int tune_tj = 4;      // theta rows per workgroup
+  int batch_wg_per_cu = 0;  // batched stage launches led by this handle: persistent, this many workgroups a CU
+  int cus = 0;              // compute units of the device
   double t = 0.0;
   std::string err;
   std::vector<void*> allocs;
@@ -554,6 +557,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
   sig.reserve(4 + 6 * (size_t)n);
   sig.push_back(n);
   sig.push_back(lead->tune_tj);
+  sig.push_back(lead->batch_wg_per_cu);
   for (int q = 0; q < n; ++q) {
     const kg_handle* h = hs[q];
     sig.push_back((int64_t)h->serial);
@@ -570,6 +574,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
     plan = &local;
     plan->sig = sig;
     plan->n = n;
+    plan->nwg = lead->batch_wg_per_cu > 0 ? (lead->cus * lead->batch_wg_per_cu + 7) / 8 * 8 : 0;
     // two consecutive steps of every member: version v is the step taken after
     // v parity advances (each parity has period 2 under advance_parity)
     std::vector<std::vector<RecOp>> rec(2 * (size_t)n);
@@ -695,7 +700,7 @@ kg_status batch_step(kg_handle* const* hs, int n, int nsteps, bool noalloc) {
         KG_HIP(lead, hipEventRecord(e0, lead->stream));
       }
       if (kind == 0) {
-        KG_HIP(lead, launch_stage_batch(mode, dsd + vo * n, dts + vo * (n + 1), n, P.total[vo],
+        KG_HIP(lead, launch_stage_batch(mode, dsd + vo * n, dts + vo * (n + 1), n, P.total[vo], P.nwg,
                                         stage_tj(mode, lead->tune_tj), ef, lead->G.fd4, lead->stream));
       } else if (kind == 2) {
         KG_HIP(lead, launch_somm_batch(mode, dsd + vo * n, dts + vo * (n + 1), n, P.total[vo], lead->G.fd4,
@@ -821,6 +826,7 @@ kg_status kg_create(const kg_desc* d, kg_handle** out) {
   G.r_out = h->hg.xmin + (double)(G.ni - 1) * h->hg.dx;  // x(Nx+g) (mesh_refinement_lib.f90:130)
 
   hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device);
   if (e == hipSuccess) e = hipStreamCreate(&h->stream);
   if (e != hipSuccess) {
     fail(nullptr, KG_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
@@ -1084,6 +1090,13 @@ kg_status kg_set_tuning(kg_handle* h, int32_t kc, int32_t tj) {
   h->tune_tj = tj;
   for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
+  return KG_OK;
+}
+
+kg_status kg_set_batch_persistent(kg_handle* h, int32_t wg_per_cu) {
+  if (!h) return fail(nullptr, KG_ERR_INVALID, "null handle");
+  if (wg_per_cu < 0 || wg_per_cu > 16) return fail(h, KG_ERR_INVALID, "batch workgroups per CU must be in [0, 16]");
+  h->batch_wg_per_cu = wg_per_cu;
   return KG_OK;
 }
 

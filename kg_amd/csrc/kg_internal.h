@@ -142,8 +142,10 @@ void finalize_tiles(StageArgs& a);
 // theta rows per workgroup a 2nd-order stage runs with for the tuned `tj`
 // (RK4 stage 4 always 4)
 int stage_tj(int mode, int tj);
-hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int tj,
-                              int ef, int fd4, hipStream_t s);
+// nwg > 0: a persistent launch of nwg workgroups (a multiple of 8) that march
+// the `total` tiles in turn; otherwise one workgroup per tile
+hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int nwg,
+                              int tj, int ef, int fd4, hipStream_t s);
 // Fill pass `phase`: grid q owns blocks [bstart[q], ...) of 256 work items, nitems[q] items.
 int64_t fill_work_items(const Geom& G, int nsys, int phase);
 // wide = 1 when some grid's pass has 2^32 items or more (64-bit index arithmetic)

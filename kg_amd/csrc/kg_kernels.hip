@@ -1076,19 +1076,21 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel(const StageArgs a) {
 template <int MODE, int EF>
 __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* __restrict__ descs,
                                                                 const int* __restrict__ tstart, int ngrids, int total) {
-  const int b = remap_xcd((int)blockIdx.x, total);
-  int lo = 0, hi = ngrids - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tstart[mid] <= b)
-      lo = mid;
-    else
-      hi = mid - 1;
+  for (int w = (int)blockIdx.x; w < total; w += (int)gridDim.x) {  // as stage_kernel_batch
+    const int b = remap_xcd(w, total);
+    int lo = 0, hi = ngrids - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tstart[mid] <= b)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const StageArgs& a = descs[lo];
+    const int local = b - tstart[lo];
+    const int nt = a.nti * a.ntj * a.ntk;
+    stage4_body<MODE, EF>(a, local % nt, local / nt);
   }
-  const StageArgs& a = descs[lo];
-  const int local = b - tstart[lo];
-  const int nt = a.nti * a.ntj * a.ntk;
-  stage4_body<MODE, EF>(a, local % nt, local / nt);
 }
 
 // One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The
@@ -1117,15 +1119,21 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ start, int n, i
   return lo;
 }
 
+// Launched with one workgroup per tile, or (persistent) with fewer: workgroup
+// w then marches tiles w, w + gridDim.x, ... in turn (every march ends on a
+// barrier, so the next tile's LDS writes follow the last reads).  gridDim.x is
+// a multiple of 8, so each workgroup stays within its XCD's run of tiles.
 template <int MODE, int EF, int TJ>
 __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel_batch(
     const StageArgs* __restrict__ descs, const int* __restrict__ tstart, int ngrids, int total) {
-  const int b = remap_xcd((int)blockIdx.x, total);
-  const int q = find_grid(tstart, ngrids, b);
-  const StageArgs& a = descs[q];
-  const int local = b - tstart[q];
-  const int nt = a.nti * a.ntj * a.ntk;
-  stage_body<MODE, EF, TJ>(a, local % nt, local / nt);
+  for (int w = (int)blockIdx.x; w < total; w += (int)gridDim.x) {
+    const int b = remap_xcd(w, total);
+    const int q = find_grid(tstart, ngrids, b);
+    const StageArgs& a = descs[q];
+    const int local = b - tstart[q];
+    const int nt = a.nti * a.ntj * a.ntk;
+    stage_body<MODE, EF, TJ>(a, local % nt, local / nt);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1485,9 +1493,13 @@ hipError_t launch_mode(const StageArgs& a, hipStream_t s, int nsys) {
   return a.tj == 8 ? launch_mode_tj<MODE, 8>(a, s, nsys) : launch_mode_tj<MODE, 4>(a, s, nsys);
 }
 
+// workgroups of a batched stage launch: one per tile, or nwg (persistent) when fewer
+unsigned batch_grid(int total, int nwg) { return (unsigned)(nwg > 0 && nwg < total ? nwg : total); }
+
 template <int MODE, int TJ>
-hipError_t launch_batch_tj(const StageArgs* descs, const int* tstart, int ngrids, int total, int ef, hipStream_t s) {
-  dim3 grid((unsigned)total, 1, 1);
+hipError_t launch_batch_tj(const StageArgs* descs, const int* tstart, int ngrids, int total, int nwg, int ef,
+                           hipStream_t s) {
+  dim3 grid(batch_grid(total, nwg), 1, 1);
   dim3 block(TI, TJ, 1);
   if (ef)
     hipLaunchKernelGGL((stage_kernel_batch<MODE, 1, TJ>), grid, block, 0, s, descs, tstart, ngrids, total);
@@ -1513,15 +1525,16 @@ void finalize_tiles(StageArgs& a) {
 }
 
 template <int MODE>
-hipError_t launch_batch_mode(const StageArgs* descs, const int* tstart, int ngrids, int total, int tj, int ef,
+hipError_t launch_batch_mode(const StageArgs* descs, const int* tstart, int ngrids, int total, int nwg, int tj, int ef,
                              hipStream_t s) {
-  return tj == 8 ? launch_batch_tj<MODE, 8>(descs, tstart, ngrids, total, ef, s)
-                 : launch_batch_tj<MODE, 4>(descs, tstart, ngrids, total, ef, s);
+  return tj == 8 ? launch_batch_tj<MODE, 8>(descs, tstart, ngrids, total, nwg, ef, s)
+                 : launch_batch_tj<MODE, 4>(descs, tstart, ngrids, total, nwg, ef, s);
 }
 
 template <int MODE>
-hipError_t launch_batch4(const StageArgs* descs, const int* tstart, int ngrids, int total, int ef, hipStream_t s) {
-  dim3 grid((unsigned)total, 1, 1);
+hipError_t launch_batch4(const StageArgs* descs, const int* tstart, int ngrids, int total, int nwg, int ef,
+                         hipStream_t s) {
+  dim3 grid(batch_grid(total, nwg), 1, 1);
   dim3 block(TI, TJ4, 1);
   if (ef)
     hipLaunchKernelGGL((stage4_kernel_batch<MODE, 1>), grid, block, 0, s, descs, tstart, ngrids, total);
@@ -1530,34 +1543,34 @@ hipError_t launch_batch4(const StageArgs* descs, const int* tstart, int ngrids, 
   return hipGetLastError();
 }
 
-hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int tj,
-                              int ef, int fd4, hipStream_t s) {
+hipError_t launch_stage_batch(int mode, const StageArgs* descs, const int* tstart, int ngrids, int total, int nwg,
+                              int tj, int ef, int fd4, hipStream_t s) {
   if (fd4) {
     switch (mode) {
-      case M_RK4_S1: return launch_batch4<M_RK4_S1>(descs, tstart, ngrids, total, ef, s);
-      case M_RK4_S2: return launch_batch4<M_RK4_S2>(descs, tstart, ngrids, total, ef, s);
-      case M_RK4_S3: return launch_batch4<M_RK4_S3>(descs, tstart, ngrids, total, ef, s);
-      case M_RK4_S4: return launch_batch4<M_RK4_S4>(descs, tstart, ngrids, total, ef, s);
-      case M_RK3_S1: return launch_batch4<M_RK3_S1>(descs, tstart, ngrids, total, ef, s);
-      case M_RK3_S2: return launch_batch4<M_RK3_S2>(descs, tstart, ngrids, total, ef, s);
-      case M_RK3_S3: return launch_batch4<M_RK3_S3>(descs, tstart, ngrids, total, ef, s);
+      case M_RK4_S1: return launch_batch4<M_RK4_S1>(descs, tstart, ngrids, total, nwg, ef, s);
+      case M_RK4_S2: return launch_batch4<M_RK4_S2>(descs, tstart, ngrids, total, nwg, ef, s);
+      case M_RK4_S3: return launch_batch4<M_RK4_S3>(descs, tstart, ngrids, total, nwg, ef, s);
+      case M_RK4_S4: return launch_batch4<M_RK4_S4>(descs, tstart, ngrids, total, nwg, ef, s);
+      case M_RK3_S1: return launch_batch4<M_RK3_S1>(descs, tstart, ngrids, total, nwg, ef, s);
+      case M_RK3_S2: return launch_batch4<M_RK3_S2>(descs, tstart, ngrids, total, nwg, ef, s);
+      case M_RK3_S3: return launch_batch4<M_RK3_S3>(descs, tstart, ngrids, total, nwg, ef, s);
       default: return hipErrorInvalidValue;  // the PIRK stages are refused at the API
     }
   }
   switch (mode) {
-    case M_RK4_S1: return launch_batch_mode<M_RK4_S1>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_RK4_S2: return launch_batch_mode<M_RK4_S2>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_RK4_S3: return launch_batch_mode<M_RK4_S3>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_RK4_S4: return launch_batch_mode<M_RK4_S4>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_RK3_S1: return launch_batch_mode<M_RK3_S1>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_RK3_S2: return launch_batch_mode<M_RK3_S2>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_RK3_S3: return launch_batch_mode<M_RK3_S3>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_PIRK_1A: return launch_batch_mode<M_PIRK_1A>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_PIRK_1B: return launch_batch_mode<M_PIRK_1B>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_PIRK2_2: return launch_batch_mode<M_PIRK2_2>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_PIRK3_2A: return launch_batch_mode<M_PIRK3_2A>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_PIRK3_2B: return launch_batch_mode<M_PIRK3_2B>(descs, tstart, ngrids, total, tj, ef, s);
-    case M_PIRK3_3: return launch_batch_mode<M_PIRK3_3>(descs, tstart, ngrids, total, tj, ef, s);
+    case M_RK4_S1: return launch_batch_mode<M_RK4_S1>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_RK4_S2: return launch_batch_mode<M_RK4_S2>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_RK4_S3: return launch_batch_mode<M_RK4_S3>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_RK4_S4: return launch_batch_mode<M_RK4_S4>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_RK3_S1: return launch_batch_mode<M_RK3_S1>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_RK3_S2: return launch_batch_mode<M_RK3_S2>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_RK3_S3: return launch_batch_mode<M_RK3_S3>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_PIRK_1A: return launch_batch_mode<M_PIRK_1A>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_PIRK_1B: return launch_batch_mode<M_PIRK_1B>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_PIRK2_2: return launch_batch_mode<M_PIRK2_2>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_PIRK3_2A: return launch_batch_mode<M_PIRK3_2A>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_PIRK3_2B: return launch_batch_mode<M_PIRK3_2B>(descs, tstart, ngrids, total, nwg, tj, ef, s);
+    case M_PIRK3_3: return launch_batch_mode<M_PIRK3_3>(descs, tstart, ngrids, total, nwg, tj, ef, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -216,6 +216,11 @@ class LinesMethod:
         """phi planes per workgroup and theta rows per workgroup of the stage kernel."""
         capi.check(self._lib.kg_set_tuning(self._h, int(kc), int(tj)), self._h)
 
+    def set_batch_persistent(self, wg_per_cu: int) -> None:
+        """Batched launches this grid leads (first member): 0 = one workgroup
+        per tile, n > 0 = one persistent launch of n workgroups per CU per stage."""
+        capi.check(self._lib.kg_set_batch_persistent(self._h, int(wg_per_cu)), self._h)
+
     def profile(self, on: bool) -> None:
         capi.check(self._lib.kg_profile_enable(self._h, 1 if on else 0), self._h)
 

@@ -107,6 +107,33 @@ def test_every_scheme_batches_like_each_member_alone(scheme, operators):
         alone[q].close()
 
 
+@pytest.mark.parametrize("scheme,operators,wg", [("rk4", "fd4", 1), ("rk4", "ref2", 2), ("pirk2", "ref2", 1),
+                                                 ("rk3", "fd4", 3)])
+def test_persistent_batch_launches_equal_each_member_alone(scheme, operators, wg):
+    """Persistent batched launches (wg workgroups per CU march all the tiles
+    in turn; with two members of the population's smallest class, 256 x 16 x
+    32, there are more tiles than workgroups): every member bit-identical to
+    its solo run, and switching back to one workgroup per tile continues
+    identically."""
+    dr = 58.2 / 256
+    specs = _specs() + [
+        MemberSpec(256, 16, 32, 0.3, 0.8, 1.44, 1.44 + 256 * dr, 20.0, 1.5, random_lm_coeffs(5, 4, 1e-3), 5),
+        MemberSpec(256, 16, 32, 0.1, 1.6, 2.88, 2.88 + 256 * dr, 30.0, 1.5, random_lm_coeffs(6, 4, 1e-3), 6),
+    ]
+    batch = [make_member(s, scheme=scheme, operators=operators) for s in specs]
+    alone = [make_member(s, scheme=scheme, operators=operators) for s in specs]
+    print("persistent batch", scheme, operators, wg, flush=True)
+    batch[0].set_batch_persistent(wg)
+    batch_step(batch, 3)
+    batch[0].set_batch_persistent(0)
+    batch_step(batch, 2)
+    for q, a in enumerate(alone):
+        a.step(5)
+        assert np.array_equal(batch[q].u_soa(), a.u_soa()), (scheme, q)
+        a.close()
+        batch[q].close()
+
+
 def test_pirk2_batch_matches_the_oracle():
     """The reference's live integrator in batched launches against the oracle."""
     specs = _specs()
