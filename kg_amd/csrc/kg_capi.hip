@@ -208,7 +208,7 @@ Box full_box(const Geom& G) { return Box{0, G.ni, 0, G.nj, 0, G.nk}; }
 Box filled_box(const Geom& G) { return Box{1, G.ni - 1, G.g + 1, G.g + G.Ny, 0, G.Nz}; }
 
 struct Roles {
-  const State5* r[R_NUM] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const State5* r[R_NUM] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 #define KG_TRY(expr)              \
@@ -278,14 +278,13 @@ kg_status stage(kg_handle* h, int mode, const Box& b, const Roles& roles) {
 }
 
 Roles roles(const State5* in, const State5* base, const State5* out, const State5* aux = nullptr,
-            const State5* kin = nullptr, const State5* aux2 = nullptr) {
+            const State5* kin = nullptr) {
   Roles r;
   r.r[R_IN] = in;
   r.r[R_BASE] = base;
   r.r[R_OUT] = out;
   r.r[R_AUX] = aux;
   r.r[R_KIN] = kin;
-  r.r[R_AUX2] = aux2;
   return r;
 }
 
@@ -316,18 +315,28 @@ kg_status fill(kg_handle* h, const State5& u, const State5& u_p) {
 }
 
 
-// rk4 (lines_method_lib.f90:424-446) with the increments k_s = rhs*dt kept
-// instead of the stage states: stage s reads u_p + c*k_{s-1} (and its ghost
-// fill) where it needs it, which is the same rounded value the reference
-// stores in u.  The new state is written over k1 (read only at its own
-// point in the last stage), and u and that buffer trade places each step.
+// rk4 (lines_method_lib.f90:424-446) on increments k_s = rhs*dt.  Stages 2
+// and 4 read u_p + c*k (and its ghost fill, the index map of boundary())
+// where they need it -- the same rounded values the reference stores in u.
+// Stage 2 stores k2 as the two values later stages read: the stage-3 state
+// Y = u_p + 0.5*k2, ghost-filled as the reference fills u (:434-435), which
+// stage 3 reads as one stored array, and C = k1 + 2*k2, the first partial
+// sum of the final combine (one array for stage 4 instead of k1 and k2).
+// Stage 4 writes the new state over C, each point reading only its own C;
+// u and that buffer trade places each step.
+//   stage 1: u_p        -> k1 (Ua)
+//   stage 2: u_p, k1    -> Y (Ub), C (X); fill Y
+//   stage 3: Y          -> k3 (Ua)
+//   stage 4: u_p, k3, C -> X (over C); fill X
 kg_status enqueue_rk4(kg_handle* h, const State5& P, const State5& X) {
   const Box fb = filled_box(h->G);
-  const State5& K1 = X;
-  KG_TRY(stage(h, M_RK4_S1, fb, roles(&P, nullptr, &K1)));
-  KG_TRY(stage(h, M_RK4_S2, fb, roles(&P, nullptr, &h->Ua, nullptr, &K1)));
-  KG_TRY(stage(h, M_RK4_S3, fb, roles(&P, nullptr, &h->Ub, nullptr, &h->Ua)));
-  KG_TRY(stage(h, M_RK4_S4, fb, roles(&P, &P, &X, &K1, &h->Ub, &h->Ua)));
+  const State5& K = h->Ua;
+  const State5& Y = h->Ub;
+  KG_TRY(stage(h, M_RK4_S1, fb, roles(&P, nullptr, &K)));
+  KG_TRY(stage(h, M_RK4_S2, fb, roles(&P, nullptr, &Y, &X, &K)));
+  KG_TRY(fill(h, Y, P));
+  KG_TRY(stage(h, M_RK4_S3, fb, roles(&Y, nullptr, &K)));
+  KG_TRY(stage(h, M_RK4_S4, fb, roles(&P, &P, &X, &X, &K)));
   KG_TRY(fill(h, X, P));
   return KG_OK;
 }

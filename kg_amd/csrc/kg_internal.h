@@ -71,9 +71,9 @@ enum StageMode : int {
   // RK4 keeps the increments k_s = R*dt and forms each stage state
   // u = u_p + c*k_s (and its ghost fill) where it reads it
   M_RK4_S1 = 0,   // k1 = R(P)dt
-  M_RK4_S2 = 1,   // k2 = R(P + 0.5k1)dt
-  M_RK4_S3 = 2,   // k3 = R(P + 0.5k2)dt
-  M_RK4_S4 = 3,   // X = P + (((k1 + 2k2) + 2k3) + R(P + k3)dt)/6
+  M_RK4_S2 = 1,   // k2 = R(P + 0.5k1)dt, stored as Y = P + 0.5k2 and C = k1 + 2k2
+  M_RK4_S3 = 2,   // k3 = R(Y)dt, Y = P + 0.5k2 stored (and ghost-filled) by stage 2
+  M_RK4_S4 = 3,   // X = P + ((C + 2k3) + R(P + k3)dt)/6, C = k1 + 2k2
   M_RK3_S1 = 4,   // U1 = P + R(P)dt
   M_RK3_S2 = 5,   // U2 = 0.75P + 0.25(U1 + R(U1)dt)
   M_RK3_S3 = 6,   // P = P/3 + 2(U2 + R(U2)dt)/3
@@ -91,13 +91,12 @@ enum StageMode : int {
 // field 4 (Pi) may sit in slot 4 or in the extra slot 5 (the PIRK stage
 // arrays keep two Pi slots).
 enum Role : int {
-  R_IN = 0,    // fields the RHS is evaluated on (RK4 stages 2-4: u_p)
-  R_KIN = 1,   // RK4 stages 2-4: increment k the stage state u_p + c*k is formed from
+  R_IN = 0,    // fields the RHS is evaluated on (RK4 stages 2 and 4: u_p; stage 3: Y = u_p + 0.5*k2)
+  R_KIN = 1,   // RK4 stages 2 and 4: increment k the stage state u_p + c*k is formed from
   R_BASE = 2,  // u_p / P / u of the combine
-  R_AUX = 3,   // RK4 k1 (stage 4), PIRK3 stage state (stage 2)
-  R_AUX2 = 4,  // RK4 k2 (stage 4)
-  R_OUT = 5,   // stage output
-  R_NUM = 6
+  R_AUX = 3,   // RK4 k1 + 2*k2 (written by stage 2, read by stage 4), PIRK3 stage state (stage 2)
+  R_OUT = 4,   // stage output
+  R_NUM = 5
 };
 
 struct StageArgs {
@@ -140,7 +139,7 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase /*0 x,1 y,2 z
 // systems (finalize_tiles sets the tile counts); any stage of any scheme.
 void finalize_tiles(StageArgs& a);
 // theta rows per workgroup a 2nd-order stage runs with for the tuned `tj`
-// (RK4 stage 4 always 4)
+// (RK4 stages 2 and 4 always 4)
 int stage_tj(int mode, int tj);
 // nwg > 0: a persistent launch of nwg workgroups (a multiple of 8) that march
 // the `total` tiles in turn; otherwise one workgroup per tile

@@ -88,8 +88,12 @@ struct Combine {
   static constexpr int hi = (MODE == M_PIRK_1A || MODE == M_PIRK3_2A) ? 4 : 5;
   static constexpr bool base = !(MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3);
   static constexpr bool aux = MODE == M_RK4_S4 || MODE == M_PIRK3_2A || MODE == M_PIRK3_2B;
-  // RK4 stages 2-4 read the stage state u_p + c*k and its ghost fill
-  static constexpr bool virt = MODE == M_RK4_S2 || MODE == M_RK4_S3 || MODE == M_RK4_S4;
+  // RK4 stages 2 and 4 read the stage state u_p + c*k and its ghost fill;
+  // stage 3 reads the state Y = u_p + k2/2 that stage 2 stored, ghost-filled
+  static constexpr bool virt = MODE == M_RK4_S2 || MODE == M_RK4_S4;
+  // stages 2 and 4 combine u_p and k at the point: the march holds both as
+  // the raw operands of its stage state
+  static constexpr bool keep = MODE == M_RK4_S2 || MODE == M_RK4_S4;
   // stages followed by a ghost fill compute only i in [1, ni-2],
   // j in [g+1, g+Ny-1], k in [0, Nz-1]: the edge rows of D_r at i = -g and
   // Nx+g, of D_theta at j = -g and Ny+g and of D_phi at k = Nz never occur
@@ -103,9 +107,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
   constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
   constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
   constexpr bool BOXED = Combine<MODE>::boxed;
-  // RK4 stage 4 combines u_p and k3 at the point: the march already holds
-  // both as the raw operands of its stage input u_p + k3
-  constexpr bool KEEP = MODE == M_RK4_S4;
+  // stages 2 and 4 keep u_p and k of their march in registers for the
+  // combine (stage 2 then needs ~164 VGPRs and runs 4 theta rows, three
+  // workgroups to a CU: 8.59 -> 8.19 ms per RK4 step on 2048 x 128 x 256,
+  // against reading u_p and k1 at the point again after the RHS)
+  constexpr bool KEEP = Combine<MODE>::keep;
 
   constexpr int W = TI + 2 * HALO_R;
   __shared__ double sG[TJ][W];    // alpha*Pi
@@ -167,9 +173,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
       if constexpr (MODE == M_RK4_S4)
         return p + k;  // u_p + k3
       else
-        return p + 0.50 * k;  // u_p + 0.5*k1, u_p + 0.5*k2
+        return p + 0.50 * k;  // u_p + 0.5*k1
     } else {
-      return gld(FP(R_IN, f), o + sysb);
+      return gld(FP(R_IN, f), o + sysb);  // u (or Y = u_p + 0.5*k2, stored by stage 2)
     }
   };
   struct Src {
@@ -413,9 +419,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
       }
     };
 
-    // combine operands and RHS terms of plane kk; vb, va, va2, vk: u_p / P / u,
-    // the PIRK3 stage state, RK4 k1, k2, k3 at the point
-    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+    // combine operands and RHS terms of plane kk; vb, va, vk: u_p / P / u,
+    // the PIRK3 stage state or RK4 k1 + 2k2, RK4 k1 or k3 at the point
+    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
     double R[5] = {0, 0, 0, 0, 0};
     double t1 = 0, t2 = 0, t5 = 0;
     double vphi = 0;
@@ -464,31 +470,30 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
       // (plane 0 resolves to plane Nz-1, so it reads plane 0 itself: the x and
       // y fills read it before the z fill overwrites it)
       const bool kept = KEEP && kk != 0;
-      if constexpr (CBASE) {
+      if constexpr (KEEP) {
+        // u_p and k at the point (stage 2: k1, stage 4: k3)
         if (kept) {
           vb[0] = vr.p;
+          vk[0] = vr.k;
 #pragma unroll
-          for (int x = 0; x < 4; ++x) vb[x == 0 ? 4 : x] = cr[x].p;
+          for (int x = 0; x < 4; ++x) {
+            vb[x == 0 ? 4 : x] = cr[x].p;
+            vk[x == 0 ? 4 : x] = cr[x].k;
+          }
         } else {
 #pragma unroll
-          for (int f = CLO; f < CHI; ++f) vb[f] = gld(FP(R_BASE, f), so);
+          for (int f = 0; f < 5; ++f) {
+            vb[f] = gld(FP(R_IN, f), so);
+            vk[f] = gld(FP(R_KIN, f), so);
+          }
         }
+      } else if constexpr (CBASE) {
+#pragma unroll
+        for (int f = CLO; f < CHI; ++f) vb[f] = gld(FP(R_BASE, f), so);
       }
       if constexpr (CAUX) {
 #pragma unroll
         for (int f = CLO; f < CHI; ++f) va[f] = gld(FP(R_AUX, f), so);
-      }
-      if constexpr (MODE == M_RK4_S4) {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) va2[f] = gld(FP(R_AUX2, f), so);
-        if (kept) {
-          vk[0] = vr.k;
-#pragma unroll
-          for (int x = 0; x < 4; ++x) vk[x == 0 ? 4 : x] = cr[x].k;
-        } else {
-#pragma unroll
-          for (int f = 0; f < 5; ++f) vk[f] = gld(FP(R_KIN, f), so);
-        }
       }
       // terms from LDS and the own registers first: they need none of the
       // loads issued before the barrier, which complete meanwhile
@@ -525,13 +530,22 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
       }
       // stage combine; `cur` is the RHS input at this point
       const double cur[5] = {vphi, psr, pst, psp, Pi};
-      if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
+      if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S3) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, R[f] * dt);  // k = rhs_u*dt (:430,433,436)
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, R[f] * dt);  // k = rhs_u*dt (:430,436)
+      } else if constexpr (MODE == M_RK4_S2) {
+        // k2 = rhs_u*dt (:433); stored as the stage-3 state u_p + 0.5*k2 and
+        // the first partial sum k1 + 2*k2 of the final combine
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          const double k2 = R[f] * dt;
+          gst(FP(R_OUT, f), so, vb[f] + 0.50 * k2);
+          gst(FP(R_AUX, f), so, vk[f] + 2.0 * k2);
+        }
       } else if constexpr (MODE == M_RK4_S4) {
-        // :439-440 u = u_p + (k1 + 2k2 + 2k3 + k4)/6, k4 = rhs_u*dt
+        // :439-440 u = u_p + (k1 + 2k2 + 2k3 + k4)/6, k4 = rhs_u*dt; va = k1 + 2k2
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * va2[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
       } else if constexpr (MODE == M_RK3_S1) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + R[f] * dt);
@@ -604,14 +618,14 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
   static_assert(Combine<MODE>::boxed && ModeTraits<MODE>::n14 && ModeTraits<MODE>::n5,
                 "the fd4 set runs the stages that are followed by a ghost fill");
   constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
-  // RK4 stage 4 combines u_p and k3 at the point: the march holds both as the
-  // raw operands of its stage input u_p + k3 (planes kk and kk+1 kept raw)
-  constexpr bool KEEP = MODE == M_RK4_S4;
-  // stages that read a stored state load the point's planes three ahead (the
-  // loads issued for plane kk+3 have a whole plane to arrive: RK4 stage 1
-  // 1.59 -> 1.40 ms on the 2048 x 128 x 256 grid); the stages that read the
-  // raw pairs (u_p, k) of a virtual stage state keep two, which measured ~3 %
-  // faster for them than three
+  // RK4 stages 2 and 4 combine u_p and k at the point: the march holds both
+  // as the raw operands of its stage state (planes kk and kk+1 kept raw)
+  constexpr bool KEEP = Combine<MODE>::keep;
+  // stages that read one stored array per field load the point's planes
+  // three ahead (the loads issued for plane kk+3 have a whole plane to
+  // arrive: RK4 stage 1 1.59 -> 1.40 ms on the 2048 x 128 x 256 grid); the
+  // stages that read the raw pairs (u_p, k) of a stage state keep two, which
+  // measured ~3 % faster for them than three
   constexpr bool PF3 = !VIRT && !KEEP;
 
   constexpr int W = TI + 2 * HR4;
@@ -915,7 +929,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
 
-    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+    double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
     double R[5] = {0, 0, 0, 0, 0};
     double t1 = 0, t2 = 0, t5 = 0;
     // x_4 on an LDS r-array: rows 1 and N-1 one-sided, central between
@@ -959,34 +973,32 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
       return (qm2 - 8.0 * qm1 + 8.0 * qp1 - qp2) * r4p;
     };
     if (act) {
-      // KEEP: u_p and k3 at the point are the raw operands of the march
+      // KEEP: u_p and k at the point are the raw operands of the march
       // (plane 0 resolves to plane Nz-1, so it reads plane 0 itself)
       const bool kept = KEEP && kk != 0;
-      if constexpr (CBASE) {
+      if constexpr (KEEP) {
         if (kept) {
           vb[0] = vr.p;
+          vk[0] = vr.k;
 #pragma unroll
-          for (int x = 0; x < 4; ++x) vb[x == 0 ? 4 : x] = cr[x].p;
+          for (int x = 0; x < 4; ++x) {
+            vb[x == 0 ? 4 : x] = cr[x].p;
+            vk[x == 0 ? 4 : x] = cr[x].k;
+          }
         } else {
 #pragma unroll
-          for (int f = 0; f < 5; ++f) vb[f] = gld(FP(R_BASE, f), so);
+          for (int f = 0; f < 5; ++f) {
+            vb[f] = gld(FP(R_IN, f), so);
+            vk[f] = gld(FP(R_KIN, f), so);
+          }
         }
+      } else if constexpr (CBASE) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) vb[f] = gld(FP(R_BASE, f), so);
       }
       if constexpr (CAUX) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) va[f] = gld(FP(R_AUX, f), so);
-      }
-      if constexpr (MODE == M_RK4_S4) {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) va2[f] = gld(FP(R_AUX2, f), so);
-        if (kept) {
-          vk[0] = vr.k;
-#pragma unroll
-          for (int x = 0; x < 4; ++x) vk[x == 0 ? 4 : x] = cr[x].k;
-        } else {
-#pragma unroll
-          for (int f = 0; f < 5; ++f) vk[f] = gld(FP(R_KIN, f), so);
-        }
       }
       R[0] = F0;
       R[1] = Dr4(sG) + be * Ar4(sPR) + dxb * psr;
@@ -1010,12 +1022,20 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
       const double t6 = am2 * vphi;
       R[4] = t1 + t2 + t3 + t4 + t5 - t6;
       const double cur[5] = {vphi, psr, pst, psp, Pi};
-      if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
+      if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S3) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, R[f] * dt);
+      } else if constexpr (MODE == M_RK4_S2) {
+        // k2 stored as the stage-3 state u_p + 0.5*k2 and as k1 + 2*k2
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          const double k2 = R[f] * dt;
+          gst(FP(R_OUT, f), so, vb[f] + 0.50 * k2);
+          gst(FP(R_AUX, f), so, vk[f] + 2.0 * k2);
+        }
       } else if constexpr (MODE == M_RK4_S4) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * va2[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
+        for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
       } else if constexpr (MODE == M_RK3_S1) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + R[f] * dt);
@@ -1094,14 +1114,14 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
 }
 
 // One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The
-// second launch bound is the minimum number of waves per SIMD.  RK4 stages 2
-// and 3 ask for 4 (128 VGPRs; 2 over their natural allocation).  The final
-// stage asks for 2 (its natural ~184 VGPRs; 3 or 4 waves spill) and always
-// runs 4 theta rows per workgroup, so two workgroups with their own barriers
-// share a CU (3.25 -> 3.14 ms against one 8-row workgroup on the
-// 2048 x 128 x 256 grid).
+// second launch bound is the minimum number of waves per SIMD.  RK4 stage 3
+// asks for 4 (128 VGPRs), stage 2 for 3 (~164 VGPRs with u_p and k1 kept for
+// its stores) and the final stage for 2 (its natural ~184 VGPRs; 3 or 4 waves
+// spill); stages 2 and 4 always run 4 theta rows per workgroup, so several
+// workgroups with their own barriers share a CU (stage 4: 3.25 -> 3.14 ms
+// against one 8-row workgroup on the 2048 x 128 x 256 grid).
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 3 : MODE == M_RK4_S3 ? 4 : 1)) void stage_kernel(const StageArgs a) {
   stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
 }
 
@@ -1124,7 +1144,7 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ start, int n, i
 // barrier, so the next tile's LDS writes follow the last reads).  gridDim.x is
 // a multiple of 8, so each workgroup stays within its XCD's run of tiles.
 template <int MODE, int EF, int TJ>
-__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : (MODE == M_RK4_S2 || MODE == M_RK4_S3) ? 4 : 1)) void stage_kernel_batch(
+__global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 3 : MODE == M_RK4_S3 ? 4 : 1)) void stage_kernel_batch(
     const StageArgs* __restrict__ descs, const int* __restrict__ tstart, int ngrids, int total) {
   for (int w = (int)blockIdx.x; w < total; w += (int)gridDim.x) {
     const int b = remap_xcd(w, total);
@@ -1355,8 +1375,14 @@ __device__ __forceinline__ void somm_item(const StageArgs& a, int64_t e) {
     R[4] = -(v * (d + P[0] / G.r_out));
   }
   const uint32_t so = OFF(N, jj, kk) + sysb;
-  double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, va2[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
-  if constexpr (CBASE) {
+  double vb[5] = {0, 0, 0, 0, 0}, va[5] = {0, 0, 0, 0, 0}, vk[5] = {0, 0, 0, 0, 0};
+  if constexpr (Combine<MODE>::keep) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      vb[f] = gld(FP(R_IN, f), so);
+      vk[f] = gld(FP(R_KIN, f), so);
+    }
+  } else if constexpr (CBASE) {
 #pragma unroll
     for (int f = CLO; f < CHI; ++f) vb[f] = gld(FP(R_BASE, f), so);
   }
@@ -1364,19 +1390,19 @@ __device__ __forceinline__ void somm_item(const StageArgs& a, int64_t e) {
 #pragma unroll
     for (int f = CLO; f < CHI; ++f) va[f] = gld(FP(R_AUX, f), so);
   }
-  if constexpr (MODE == M_RK4_S4) {
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      va2[f] = gld(FP(R_AUX2, f), so);
-      vk[f] = gld(FP(R_KIN, f), so);
-    }
-  }
-  if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S2 || MODE == M_RK4_S3) {
+  if constexpr (MODE == M_RK4_S1 || MODE == M_RK4_S3) {
 #pragma unroll
     for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, R[f] * dt);
+  } else if constexpr (MODE == M_RK4_S2) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      const double k2 = R[f] * dt;
+      gst(FP(R_OUT, f), so, vb[f] + 0.50 * k2);
+      gst(FP(R_AUX, f), so, vk[f] + 2.0 * k2);
+    }
   } else if constexpr (MODE == M_RK4_S4) {
 #pragma unroll
-    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * va2[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
+    for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + (va[f] + 2.0 * vk[f] + R[f] * dt) / 6.0);
   } else if constexpr (MODE == M_RK3_S1) {
 #pragma unroll
     for (int f = 0; f < 5; ++f) gst(FP(R_OUT, f), so, vb[f] + R[f] * dt);
@@ -1510,7 +1536,7 @@ hipError_t launch_batch_tj(const StageArgs* descs, const int* tstart, int ngrids
 
 }  // namespace
 
-int stage_tj(int mode, int tj) { return mode == M_RK4_S4 ? 4 : tj; }
+int stage_tj(int mode, int tj) { return (mode == M_RK4_S4 || mode == M_RK4_S2) ? 4 : tj; }
 
 void finalize_tiles(StageArgs& a) {
   if (a.tj != 4 && a.tj != 8) a.tj = 4;
