@@ -71,6 +71,20 @@ __device__ __forceinline__ int remap_xcd(int b, int nb) {
   return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
+__device__ __forceinline__ int deal_theta(int tj, int ntj) {
+  // After remap_xcd, XCD x holds theta tiles [x*q, (x+1)*q) (q = ntj / NXCD,
+  // when that divides) and all XCDs run the same position of their runs at
+  // about the same time.  Dealing the theta tiles out instead (XCD x runs
+  // x, x + NXCD, ...) has the XCDs march theta-neighbouring tiles side by
+  // side, so the halo rows a tile reads are its neighbours' own rows, read
+  // at about the same time and served from the memory-side cache rather
+  // than HBM (fd4 RK4 on 2048 x 128 x 256: 8.96 -> 8.53 ms per step).
+  // Bijective on [0, ntj).
+  if (ntj % NXCD != 0) return tj;
+  const int q = ntj / NXCD;
+  return (tj % q) * NXCD + tj / q;
+}
+
 // Stage kernel.  A workgroup owns a TI x TJ tile of (r, theta) columns and
 // marches over a chunk of phi planes.  At each plane every thread loads its
 // point's five fields, forms the products the stencils act on (alpha*Pi,
@@ -101,7 +115,7 @@ struct Combine {
 };
 
 template <int MODE, int EF, int TJ>
-__device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, const int sysidx) {
+__device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, const int sysidx, const bool xil) {
   constexpr bool N14 = ModeTraits<MODE>::n14;
   constexpr bool N5 = ModeTraits<MODE>::n5;
   constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
@@ -141,6 +155,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
       tj = full * P + r3 % rem;
       tk = r3 / rem;
     }
+    if (xil) tj = deal_theta(tj, a.ntj);
   }
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
   const int tid = row * TI + lane;
@@ -612,8 +627,12 @@ constexpr int HR4 = 3;  // r halo of the fd4 set
 // stages measured no faster on the full grid)
 constexpr int TJ4 = 8;
 
-template <int MODE, int EF>
-__device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, const int sysidx) {
+// EDGE: the tile's r range reaches a fixed row of x_4 (1, N-1) or of
+// advec_x_4 (1, 2, N-2, N-1), whose stencils then test the row per point;
+// interior tiles run the march without those tests (~2 % of an RK4 step on
+// 2048 x 128 x 256)
+template <int MODE, int EF, bool EDGE>
+__device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, const int sysidx, const bool xil) {
   constexpr int TJ = TJ4;
   static_assert(Combine<MODE>::boxed && ModeTraits<MODE>::n14 && ModeTraits<MODE>::n5,
                 "the fd4 set runs the stages that are followed by a ghost fill");
@@ -656,6 +675,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
       tj = full * P + r3 % rem;
       tk = r3 / rem;
     }
+    if (xil) tj = deal_theta(tj, a.ntj);
   }
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
   const int tid = row * TI + lane;
@@ -934,30 +954,34 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
     double t1 = 0, t2 = 0, t5 = 0;
     // x_4 on an LDS r-array: rows 1 and N-1 one-sided, central between
     auto Dr4 = [&](const double(*A)[W]) {
-      if (ii == NR - 1)
-        return (+3.0 * A[row][c + 1] + 10.0 * A[row][c] - 18.0 * A[row][c - 1] + 6.0 * A[row][c - 2] -
-                1.0 * A[row][c - 3]) * r4r;
-      if (ii == 1)
-        return (-3.0 * A[row][c - 1] - 10.0 * A[row][c] + 18.0 * A[row][c + 1] - 6.0 * A[row][c + 2] +
-                1.0 * A[row][c + 3]) * r4r;
+      if constexpr (EDGE) {
+        if (ii == NR - 1)
+          return (+3.0 * A[row][c + 1] + 10.0 * A[row][c] - 18.0 * A[row][c - 1] + 6.0 * A[row][c - 2] -
+                  1.0 * A[row][c - 3]) * r4r;
+        if (ii == 1)
+          return (-3.0 * A[row][c - 1] - 10.0 * A[row][c] + 18.0 * A[row][c + 1] - 6.0 * A[row][c + 2] +
+                  1.0 * A[row][c + 3]) * r4r;
+      }
       return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * r4r;
     };
     // advec_x_4 at the point: its fixed rows 1, 2, N-2, N-1, upwinded between
     auto Ar4 = [&](const double(*A)[W]) {
-      if (ii == NR - 1)
-        return (3.0 * A[row][c + 1] + 10.0 * A[row][c] - 18.0 * A[row][c - 1] + 6.0 * A[row][c - 2] -
-                A[row][c - 3]) * rar / 12.0;
-      if (ii == NR - 2 || ii == 2)
-        return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * rar / 12.0;
-      if (ii == 1)
+      auto plus = [&]() {  // beta > 0
         return (-3.0 * A[row][c - 1] - 10.0 * A[row][c] + 18.0 * A[row][c + 1] - 6.0 * A[row][c + 2] +
                 A[row][c + 3]) * rar / 12.0;
-      if (be < 0.0)
+      };
+      auto minus = [&]() {  // beta < 0
         return (3.0 * A[row][c + 1] + 10.0 * A[row][c] - 18.0 * A[row][c - 1] + 6.0 * A[row][c - 2] -
                 A[row][c - 3]) * rar / 12.0;
-      if (be > 0.0)
-        return (-3.0 * A[row][c - 1] - 10.0 * A[row][c] + 18.0 * A[row][c + 1] - 6.0 * A[row][c + 2] +
-                A[row][c + 3]) * rar / 12.0;
+      };
+      if constexpr (EDGE) {
+        if (ii == NR - 1) return minus();
+        if (ii == NR - 2 || ii == 2)
+          return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * rar / 12.0;
+        if (ii == 1) return plus();
+      }
+      if (be < 0.0) return minus();
+      if (be > 0.0) return plus();
       return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * rar / 12.0;
     };
     // y_4 away from its edge rows (the computed rows are all central)
@@ -1087,9 +1111,19 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
   }
 }
 
+// a workgroup-uniform choice between the edge and the interior march
+template <int MODE, int EF>
+__device__ __forceinline__ void stage4_run(const StageArgs& a, const int tile, const int sysidx, const bool xil) {
+  const int i0 = a.i_lo + (tile % a.nti) * TI, NR = a.G.ni - 1;
+  if (i0 <= 2 || i0 + TI - 1 >= NR - 2)
+    stage4_body<MODE, EF, true>(a, tile, sysidx, xil);
+  else
+    stage4_body<MODE, EF, false>(a, tile, sysidx, xil);
+}
+
 template <int MODE, int EF>
 __global__ __launch_bounds__(TI* TJ4) void stage4_kernel(const StageArgs a) {
-  stage4_body<MODE, EF>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
+  stage4_run<MODE, EF>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y, true);
 }
 
 // Many fd4 grids in one launch (as stage_kernel_batch).
@@ -1109,7 +1143,7 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
     const StageArgs& a = descs[lo];
     const int local = b - tstart[lo];
     const int nt = a.nti * a.ntj * a.ntk;
-    stage4_body<MODE, EF>(a, local % nt, local / nt);
+    stage4_run<MODE, EF>(a, local % nt, local / nt, false);
   }
 }
 
@@ -1122,7 +1156,7 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
 // against one 8-row workgroup on the 2048 x 128 x 256 grid).
 template <int MODE, int EF, int TJ>
 __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 3 : MODE == M_RK4_S3 ? 4 : 1)) void stage_kernel(const StageArgs a) {
-  stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y);
+  stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y, true);
 }
 
 // Many grids in one launch: grid q owns the workgroups [tstart[q], tstart[q+1])
@@ -1152,7 +1186,7 @@ __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 
     const StageArgs& a = descs[q];
     const int local = b - tstart[q];
     const int nt = a.nti * a.ntj * a.ntk;
-    stage_body<MODE, EF, TJ>(a, local % nt, local / nt);
+    stage_body<MODE, EF, TJ>(a, local % nt, local / nt, false);
   }
 }
 
