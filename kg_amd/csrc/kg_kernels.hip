@@ -52,6 +52,11 @@ __device__ __forceinline__ void gst(double* base, uint32_t off, double v) {
 }
 
 constexpr int TI = 64;  // tile width along r: one wavefront per theta row
+// theta tiles per group of tile_in_run: the 2nd-order stages take the phi
+// chunks of two theta tiles in turn, the fd4 stages one tile at a time
+// (pairs measured 1.2 % slower there: 9.75 against 9.63 ms per RK4 step,
+// and 8.64 against 8.52 with the theta tiles dealt over the XCDs)
+constexpr int P2 = 2, P4 = 1;
 constexpr int HALO_R = 2;  // r-neighbours needed on each side (A_r reaches i-2 and i+2)
 constexpr int NXCD = 8;    // XCDs of an MI355X; workgroups are dealt round-robin over them
 
@@ -71,18 +76,54 @@ __device__ __forceinline__ int remap_xcd(int b, int nb) {
   return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
-__device__ __forceinline__ int deal_theta(int tj, int ntj) {
-  // After remap_xcd, XCD x holds theta tiles [x*q, (x+1)*q) (q = ntj / NXCD,
-  // when that divides) and all XCDs run the same position of their runs at
-  // about the same time.  Dealing the theta tiles out instead (XCD x runs
-  // x, x + NXCD, ...) has the XCDs march theta-neighbouring tiles side by
-  // side, so the halo rows a tile reads are its neighbours' own rows, read
-  // at about the same time and served from the memory-side cache rather
-  // than HBM (fd4 RK4 on 2048 x 128 x 256: 8.96 -> 8.53 ms per step).
-  // Bijective on [0, ntj).
-  if (ntj % NXCD != 0) return tj;
+struct Tile {
+  int ti, tj, tk;  // r tile, theta tile, phi chunk
+};
+
+// The order of the tiles in a run of workgroups: r tiles fastest (the r
+// neighbours share halo columns in L2), then the phi chunks of a group of P
+// theta tiles, taken in turn (chunk 0 of each tile of the group, then chunk
+// 1, ...), then the groups; a last theta group of fewer tiles comes last.
+__device__ __forceinline__ Tile tile_in_run(int t, int nti, int ntj, int ntk, int P) {
+  Tile o;
+  o.ti = t % nti;
+  const int rest = t / nti;
+  const int full = ntj / P;
+  const int tiles_full = full * P * ntk;
+  if (rest < tiles_full) {
+    const int g2 = rest / (P * ntk), r2 = rest % (P * ntk);
+    o.tj = g2 * P + r2 % P;
+    o.tk = r2 / P;
+  } else {
+    const int r3 = rest - tiles_full, rem = ntj - full * P;
+    o.tj = full * P + r3 % rem;
+    o.tk = r3 / rem;
+  }
+  return o;
+}
+
+// Tile of workgroup b of a one-grid launch.  Workgroups are dealt
+// round-robin over the XCDs, and all XCDs reach the same position of their
+// share at about the same time.  The theta tiles are dealt the same way:
+// with q = ntj / NXCD, XCD x runs theta tiles x, x + NXCD, ..., x +
+// (q-1)*NXCD (in the order of tile_in_run), so the XCDs march
+// theta-neighbouring tiles side by side and the halo rows a tile reads are
+// its neighbours' own rows, read at about the same time and served from the
+// memory-side cache rather than HBM (fd4 RK4 on 2048 x 128 x 256: 8.96 ->
+// 8.53 ms per step).  The ntj % NXCD theta tiles left over follow, split
+// into contiguous runs (remap_xcd).  Bijective on [0, nti*ntj*ntk).
+__device__ __forceinline__ Tile tile_dealt(int b, int nti, int ntj, int ntk, int P) {
   const int q = ntj / NXCD;
-  return (tj % q) * NXCD + tj / q;
+  const int nA = NXCD * q * nti * ntk;
+  if (b < nA) {
+    Tile o = tile_in_run(b / NXCD, nti, q, ntk, P);
+    o.tj = o.tj * NXCD + b % NXCD;
+    return o;
+  }
+  const int rem = ntj - NXCD * q;
+  Tile o = tile_in_run(remap_xcd(b - nA, rem * nti * ntk), nti, rem, ntk, P);
+  o.tj += NXCD * q;
+  return o;
 }
 
 // Stage kernel.  A workgroup owns a TI x TJ tile of (r, theta) columns and
@@ -115,7 +156,7 @@ struct Combine {
 };
 
 template <int MODE, int EF, int TJ>
-__device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, const int sysidx, const bool xil) {
+__device__ __forceinline__ void stage_body(const StageArgs& a, const Tile tile, const int sysidx) {
   constexpr bool N14 = ModeTraits<MODE>::n14;
   constexpr bool N5 = ModeTraits<MODE>::n5;
   constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
@@ -136,27 +177,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile, c
   __shared__ double sFT[TJ + 2][TI];  // theta-flux of rhs_pi
 
   const Geom& G = a.G;
-  // r tiles fastest, then phi chunks of pairs of theta rows, then the pairs:
-  // tiles that follow each other on one XCD share the phi planes at their
-  // chunk edges and one theta halo row of the pair in its L2
-  const int ti = tile % a.nti;
-  int tj, tk;
-  {
-    constexpr int P = 2;
-    const int rest = tile / a.nti;
-    const int full = a.ntj / P;
-    const int tiles_full = full * P * a.ntk;
-    if (rest < tiles_full) {
-      const int g2 = rest / (P * a.ntk), r2 = rest % (P * a.ntk);
-      tj = g2 * P + r2 % P;
-      tk = r2 / P;
-    } else {  // an odd last theta row
-      const int r3 = rest - tiles_full, rem = a.ntj - full * P;
-      tj = full * P + r3 % rem;
-      tk = r3 / rem;
-    }
-    if (xil) tj = deal_theta(tj, a.ntj);
-  }
+  const int ti = tile.ti, tj = tile.tj, tk = tile.tk;
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
   const int tid = row * TI + lane;
   const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
@@ -632,7 +653,7 @@ constexpr int TJ4 = 8;
 // interior tiles run the march without those tests (~2 % of an RK4 step on
 // 2048 x 128 x 256)
 template <int MODE, int EF, bool EDGE>
-__device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, const int sysidx, const bool xil) {
+__device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile, const int sysidx) {
   constexpr int TJ = TJ4;
   static_assert(Combine<MODE>::boxed && ModeTraits<MODE>::n14 && ModeTraits<MODE>::n5,
                 "the fd4 set runs the stages that are followed by a ghost fill");
@@ -656,27 +677,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
   __shared__ double sFT[TJ + 4][TI];  // theta-flux of rhs_pi
 
   const Geom& G = a.G;
-  const int ti = tile % a.nti;
-  int tj, tk;
-  {
-    // tiles of one r row, then the phi chunks of one theta tile (their
-    // shared edge planes meet in L2); pairing theta tiles as the 2nd-order
-    // stages do measured 1.2 % slower here (9.75 against 9.63 ms per step)
-    constexpr int P = 1;
-    const int rest = tile / a.nti;
-    const int full = a.ntj / P;
-    const int tiles_full = full * P * a.ntk;
-    if (rest < tiles_full) {
-      const int g2 = rest / (P * a.ntk), r2 = rest % (P * a.ntk);
-      tj = g2 * P + r2 % P;
-      tk = r2 / P;
-    } else {
-      const int r3 = rest - tiles_full, rem = a.ntj - full * P;
-      tj = full * P + r3 % rem;
-      tk = r3 / rem;
-    }
-    if (xil) tj = deal_theta(tj, a.ntj);
-  }
+  const int ti = tile.ti, tj = tile.tj, tk = tile.tk;
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
   const int tid = row * TI + lane;
   const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
@@ -1113,17 +1114,17 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const int tile, 
 
 // a workgroup-uniform choice between the edge and the interior march
 template <int MODE, int EF>
-__device__ __forceinline__ void stage4_run(const StageArgs& a, const int tile, const int sysidx, const bool xil) {
-  const int i0 = a.i_lo + (tile % a.nti) * TI, NR = a.G.ni - 1;
+__device__ __forceinline__ void stage4_run(const StageArgs& a, const Tile tile, const int sysidx) {
+  const int i0 = a.i_lo + tile.ti * TI, NR = a.G.ni - 1;
   if (i0 <= 2 || i0 + TI - 1 >= NR - 2)
-    stage4_body<MODE, EF, true>(a, tile, sysidx, xil);
+    stage4_body<MODE, EF, true>(a, tile, sysidx);
   else
-    stage4_body<MODE, EF, false>(a, tile, sysidx, xil);
+    stage4_body<MODE, EF, false>(a, tile, sysidx);
 }
 
 template <int MODE, int EF>
 __global__ __launch_bounds__(TI* TJ4) void stage4_kernel(const StageArgs a) {
-  stage4_run<MODE, EF>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y, true);
+  stage4_run<MODE, EF>(a, tile_dealt((int)blockIdx.x, a.nti, a.ntj, a.ntk, P4), (int)blockIdx.y);
 }
 
 // Many fd4 grids in one launch (as stage_kernel_batch).
@@ -1143,7 +1144,7 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
     const StageArgs& a = descs[lo];
     const int local = b - tstart[lo];
     const int nt = a.nti * a.ntj * a.ntk;
-    stage4_run<MODE, EF>(a, local % nt, local / nt, false);
+    stage4_run<MODE, EF>(a, tile_in_run(local % nt, a.nti, a.ntj, a.ntk, P4), local / nt);
   }
 }
 
@@ -1156,7 +1157,7 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
 // against one 8-row workgroup on the 2048 x 128 x 256 grid).
 template <int MODE, int EF, int TJ>
 __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 3 : MODE == M_RK4_S3 ? 4 : 1)) void stage_kernel(const StageArgs a) {
-  stage_body<MODE, EF, TJ>(a, remap_xcd((int)blockIdx.x, a.nti * a.ntj * a.ntk), (int)blockIdx.y, true);
+  stage_body<MODE, EF, TJ>(a, tile_dealt((int)blockIdx.x, a.nti, a.ntj, a.ntk, P2), (int)blockIdx.y);
 }
 
 // Many grids in one launch: grid q owns the workgroups [tstart[q], tstart[q+1])
@@ -1186,7 +1187,7 @@ __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 
     const StageArgs& a = descs[q];
     const int local = b - tstart[q];
     const int nt = a.nti * a.ntj * a.ntk;
-    stage_body<MODE, EF, TJ>(a, local % nt, local / nt, false);
+    stage_body<MODE, EF, TJ>(a, tile_in_run(local % nt, a.nti, a.ntj, a.ntk, P2), local / nt);
   }
 }
 
