@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-CANDIDATES = [(kc, tj) for tj in (4, 8) for kc in (8, 16, 32, 64)]
+CANDIDATES = [(kc, tj) for tj in (4, 8) for kc in (4, 8, 16, 32, 64)]
 
 
 def autotune(lm, candidates=CANDIDATES, steps=2, verbose=False):

@@ -139,7 +139,7 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s, int phase /*0 x,1 y,2 z
 // systems (finalize_tiles sets the tile counts); any stage of any scheme.
 void finalize_tiles(StageArgs& a);
 // theta rows per workgroup a 2nd-order stage runs with for the tuned `tj`
-// (RK4 stages 2 and 4 always 4)
+// (RK4 stages 2 and 4 and the PIRK stages of Pi alone always 4)
 int stage_tj(int mode, int tj);
 // nwg > 0: a persistent launch of nwg workgroups (a multiple of 8) that march
 // the `total` tiles in turn; otherwise one workgroup per tile
