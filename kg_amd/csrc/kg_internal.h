@@ -109,6 +109,7 @@ struct StageArgs {
   int kc;                   // phi planes per workgroup
   int tj;                   // theta rows per workgroup (4 or 8)
   int nti, ntj, ntk;        // tile counts (set by launch_stage)
+  int nts;                  // theta strips of the fd4 set's last r columns (0: none; launch_stage)
   int nsys;                 // independent real systems sharing the grid
 };
 

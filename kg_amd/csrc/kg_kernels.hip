@@ -648,13 +648,33 @@ constexpr int HR4 = 3;  // r halo of the fd4 set
 // stages measured no faster on the full grid)
 constexpr int TJ4 = 8;
 
+// The last r columns of a box that is not a whole number of 64-point tiles
+// wide (2051 = 32 x 64 + 3 on the 2048-point grid) run as strips of SI4 x
+// SJ4 points instead of 64 x 8 tiles with a few active lanes: one strip
+// covers 128 theta rows, where the 64-wide tiles of those columns took 16
+// workgroups.  On 2048 x 128 x 256 those tiles cost 5.8 % of an RK4 step;
+// the strips take 8.59 -> 8.47 ms per step back (a strip's waves span 16
+// theta rows, each lane with its own fixed r row, so its march is several
+// times slower than a tile's).  A strip needs no right r halo (its columns
+// reach ni-1), so it is used when its r range holds at most SI4 - 1
+// columns of the box.
+constexpr int SI4 = 4, SJ4 = 128;
+
 // EDGE: the tile's r range reaches a fixed row of x_4 (1, N-1) or of
 // advec_x_4 (1, 2, N-2, N-1), whose stencils then test the row per point;
 // interior tiles run the march without those tests (~2 % of an RK4 step on
-// 2048 x 128 x 256)
-template <int MODE, int EF, bool EDGE>
-__device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile, const int sysidx) {
-  constexpr int TJ = TJ4;
+// 2048 x 128 x 256).  TIL x TJ: the tile's points in r and theta (64 x 8,
+// or a strip's SI4 x SJ4), on the same 512 threads.
+template <int TIL, int TJ>
+constexpr int lds4_doubles() {  // LDS of a tile: 4 r-arrays with halos, 2 theta-arrays with halos
+  return 4 * TJ * (TIL + 2 * HR4) + 2 * (TJ + 4) * TIL;
+}
+constexpr int LDS4 = lds4_doubles<TI, TJ4>() > lds4_doubles<SI4, SJ4>() ? lds4_doubles<TI, TJ4>()
+                                                                        : lds4_doubles<SI4, SJ4>();
+
+template <int MODE, int EF, bool EDGE, int TIL = TI, int TJ = TJ4>
+__device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile, const int sysidx, double* lds) {
+  static_assert(TIL * TJ == TI * TJ4, "a tile has one point per thread");
   static_assert(Combine<MODE>::boxed && ModeTraits<MODE>::n14 && ModeTraits<MODE>::n5,
                 "the fd4 set runs the stages that are followed by a ghost fill");
   constexpr bool CBASE = Combine<MODE>::base, CAUX = Combine<MODE>::aux, VIRT = Combine<MODE>::virt;
@@ -668,18 +688,23 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile,
   // measured ~3 % faster for them than three
   constexpr bool PF3 = !VIRT && !KEEP;
 
-  constexpr int W = TI + 2 * HR4;
-  __shared__ double sG[TJ][W];        // alpha*Pi
-  __shared__ double sPR[TJ][W];       // psi_r
-  __shared__ double sFR[TJ][W];       // r-flux of rhs_pi
-  __shared__ double sPS[TJ][W];       // Pi*sqrtg
-  __shared__ double sF[TJ + 4][TI];   // alpha*Pi + beta*psi_r
-  __shared__ double sFT[TJ + 4][TI];  // theta-flux of rhs_pi
+  // the tile's LDS arrays, carved from one workgroup buffer that every tile
+  // shape shares
+  constexpr int W = TIL + 2 * HR4;
+  static_assert(lds4_doubles<TIL, TJ>() <= LDS4, "LDS buffer too small");
+  double(*const sG)[W] = reinterpret_cast<double(*)[W]>(lds);               // alpha*Pi
+  double(*const sPR)[W] = reinterpret_cast<double(*)[W]>(lds + TJ * W);     // psi_r
+  double(*const sFR)[W] = reinterpret_cast<double(*)[W]>(lds + 2 * TJ * W);  // r-flux of rhs_pi
+  double(*const sPS)[W] = reinterpret_cast<double(*)[W]>(lds + 3 * TJ * W);  // Pi*sqrtg
+  double(*const sF)[TIL] = reinterpret_cast<double(*)[TIL]>(lds + 4 * TJ * W);  // alpha*Pi + beta*psi_r
+  double(*const sFT)[TIL] =
+      reinterpret_cast<double(*)[TIL]>(lds + 4 * TJ * W + (TJ + 4) * TIL);  // theta-flux of rhs_pi
 
   const Geom& G = a.G;
   const int ti = tile.ti, tj = tile.tj, tk = tile.tk;
-  const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
-  const int tid = row * TI + lane;
+  const int tid = (int)threadIdx.y * TI + (int)threadIdx.x;
+  const int lane = tid % TIL, row = tid / TIL;
+  // (a strip's ti is the count of whole r tiles: it starts where they end)
   const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
   const int ii = i0 + lane, jj = j0 + row;
   const int ni = G.ni, nj = G.nj, Nz = G.Nz, Ny = G.Ny, g = G.g, ld = G.ld;
@@ -785,22 +810,24 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile,
   };
 
   // halo task: 1 = theta row (j0-2, j0-1, j0+TJ, j0+TJ+1), 2 = r column
-  // (i0-3..i0-1, i0+TI..i0+TI+2), 0 = none
+  // (i0-3..i0-1, i0+TIL..i0+TIL+2; a strip has the left three only), 0 = none
+  constexpr int RH = TIL == TI ? 6 : 3;  // r-halo columns per row
+  static_assert(4 * TIL + RH * TJ <= TI * TJ4, "one halo task per thread at most");
   int hkind = 0, hi = 0, hj = 0, hrow = 0, hcol = 0;
-  if (tid < 4 * TI) {
+  if (tid < 4 * TIL) {
     hkind = 1;
-    const int r4 = tid / TI;
-    hi = i0 + (tid % TI);
+    const int r4 = tid / TIL;
+    hi = i0 + (tid % TIL);
     hrow = r4 < 2 ? r4 : TJ + r4;
     hj = j0 - 2 + hrow;
-    hcol = tid % TI;
-  } else if (tid < 4 * TI + 6 * TJ) {
-    const int u = tid - 4 * TI;
-    const int cc = u % 6;
+    hcol = tid % TIL;
+  } else if (tid < 4 * TIL + RH * TJ) {
+    const int u = tid - 4 * TIL;
+    const int cc = u % RH;
     hkind = 2;
-    hrow = u / 6;
+    hrow = u / RH;
     hj = j0 + hrow;
-    hi = cc < 3 ? i0 - 3 + cc : i0 + TI + (cc - 3);
+    hi = cc < 3 ? i0 - 3 + cc : i0 + TIL + (cc - 3);
     hcol = hi - i0 + HR4;
   }
   if (hi < 0 || hi >= ni || hj < 0 || hj >= nj) hkind = 0;
@@ -986,7 +1013,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile,
       return (A[row][c - 2] - 8.0 * A[row][c - 1] + 8.0 * A[row][c + 1] - A[row][c + 2]) * rar / 12.0;
     };
     // y_4 away from its edge rows (the computed rows are all central)
-    auto Dth4 = [&](const double(*A)[TI]) {
+    auto Dth4 = [&](const double(*A)[TIL]) {
       return (A[row][lane] - 8.0 * A[row + 1][lane] + 8.0 * A[row + 3][lane] - A[row + 4][lane]) * r4t;
     };
     // z_4: one-sided rows 0, 1 and Nz-1; planes beyond the queue from `far`
@@ -1112,19 +1139,30 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile,
   }
 }
 
-// a workgroup-uniform choice between the edge and the interior march
+// a workgroup-uniform choice between a strip and the edge and interior
+// marches of the 64 x 8 tiles
 template <int MODE, int EF>
 __device__ __forceinline__ void stage4_run(const StageArgs& a, const Tile tile, const int sysidx) {
+  __shared__ double lds[LDS4];
+  if (tile.ti == a.nti) {
+    stage4_body<MODE, EF, true, SI4, SJ4>(a, tile, sysidx, lds);
+    return;
+  }
   const int i0 = a.i_lo + tile.ti * TI, NR = a.G.ni - 1;
   if (i0 <= 2 || i0 + TI - 1 >= NR - 2)
-    stage4_body<MODE, EF, true>(a, tile, sysidx);
+    stage4_body<MODE, EF, true>(a, tile, sysidx, lds);
   else
-    stage4_body<MODE, EF, false>(a, tile, sysidx);
+    stage4_body<MODE, EF, false>(a, tile, sysidx, lds);
 }
 
+// One grid: workgroups [0, nts*ntk) run the strips (theta strip fastest;
+// first, so their longer marches overlap the tiles'), the nti*ntj*ntk after
+// them the 64 x 8 tiles, dealt over the XCDs.
 template <int MODE, int EF>
 __global__ __launch_bounds__(TI* TJ4) void stage4_kernel(const StageArgs a) {
-  stage4_run<MODE, EF>(a, tile_dealt((int)blockIdx.x, a.nti, a.ntj, a.ntk, P4), (int)blockIdx.y);
+  const int b = (int)blockIdx.x, ns = a.nts * a.ntk;
+  const Tile t = b < ns ? Tile{a.nti, b % a.nts, b / a.nts} : tile_dealt(b - ns, a.nti, a.ntj, a.ntk, P4);
+  stage4_run<MODE, EF>(a, t, (int)blockIdx.y);
 }
 
 // Many fd4 grids in one launch (as stage_kernel_batch).
@@ -1529,7 +1567,14 @@ hipError_t launch_fd4(const StageArgs& a0, hipStream_t s, int nsys) {
   StageArgs a = a0;
   a.tj = TJ4;
   finalize_tiles(a);
-  dim3 grid((unsigned)(a.nti * a.ntj * a.ntk), (unsigned)nsys, 1);
+  // the last r tile as strips when its columns end at ni-1 (no right halo)
+  // and fewer than SI4 of them are in the box
+  const int i_last = a.i_lo + (a.nti - 1) * TI;
+  if (a.nti > 1 && a.i_hi == a.G.ni - 1 && a.i_hi - i_last < SI4) {
+    a.nti -= 1;
+    a.nts = (a.j_hi - a.j_lo + SJ4 - 1) / SJ4;
+  }
+  dim3 grid((unsigned)((a.nti * a.ntj + a.nts) * a.ntk), (unsigned)nsys, 1);
   dim3 block(TI, TJ4, 1);
   if (a.G.metric_ef)
     hipLaunchKernelGGL((stage4_kernel<MODE, 1>), grid, block, 0, s, a);
@@ -1584,6 +1629,7 @@ void finalize_tiles(StageArgs& a) {
   if (a.kc < 1) a.kc = 1;
   a.nti = (a.i_hi - a.i_lo + TI - 1) / TI;
   a.ntj = (a.j_hi - a.j_lo + a.tj - 1) / a.tj;
+  a.nts = 0;
   a.ntk = (a.k_hi - a.k_lo + a.kc - 1) / a.kc;
   // the same number of chunks, balanced: 257 planes at kc = 32 march as 9
   // chunks of 29 instead of 8 of 32 and one of a single plane

@@ -32,6 +32,16 @@ CASES = [
                           bc="outflow")),
     ("tall_theta", dict(Nx=70, Ny=37, Nz=12, xmin=1.8, xmax=15.0, metric="Kerr_Schild", a=0.5, M_bh=1.0, m=0.2,
                         bc="sommerfeld")),
+    # boxes of 64k + 1, 2, 3 r columns: the last columns run as 4-wide strips
+    # of 128 theta rows (two strips on the 139-row box)
+    ("r_strip_1col", dict(Nx=126, Ny=9, Nz=9, xmin=1.8, xmax=25.0, metric="Kerr_Schild", a=0.4, M_bh=1.0, m=0.2,
+                          bc="outflow")),
+    ("r_strip_2col", dict(Nx=127, Ny=8, Nz=10, xmin=1.8, xmax=25.0, metric="Kerr_Schild", a=0.0, M_bh=1.0, m=0.1,
+                          bc="inflow")),
+    ("r_strip_3col", dict(Nx=128, Ny=10, Nz=9, xmin=2.5, xmax=25.0, metric="Schw_EF", a=0.0, M_bh=1.0, m=0.2,
+                          bc="sommerfeld")),
+    ("r_strips_tall", dict(Nx=64, Ny=140, Nz=7, xmin=1.8, xmax=12.0, metric="Kerr_Schild", a=0.5, M_bh=1.0, m=0.3,
+                           bc="sommerfeld")),
 ]
 
 
@@ -77,7 +87,7 @@ def test_fd4_steps_match_oracle(scheme, name, p):
 
 
 @pytest.mark.parametrize("kc,tj", [(1, 4), (2, 8), (3, 8), (64, 8)])
-@pytest.mark.parametrize("case", [1, 6])
+@pytest.mark.parametrize("case", [1, 6, 9])
 def test_fd4_launch_shape_does_not_change_results(kc, tj, case):
     """The fd4 stages run tiles of 8 theta rows whatever tj; the phi chunk
     varies (1, 2, 3 planes and a whole column)."""
@@ -92,8 +102,9 @@ def test_fd4_launch_shape_does_not_change_results(kc, tj, case):
     gpu.close()
 
 
-def test_fd4_complex_state():
-    p = CASES[0][1]
+@pytest.mark.parametrize("case", [0, 10])
+def test_fd4_complex_state(case):
+    p = CASES[case][1]
     sim, gpu, ure = _pair(p, "rk4", seed=3, is_complex=True)
     uim = shell_state(p["Nx"], p["Ny"], p["Nz"], 2, p["xmin"], p["xmax"], 6.0, 1.0, 0.3, {(1, 1): 1.0})
     gpu.set_u_soa(np.stack([ure, uim]))
