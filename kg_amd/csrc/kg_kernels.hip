@@ -703,7 +703,8 @@ __device__ __forceinline__ void stage4_body(const StageArgs& a, const Tile tile,
   const Geom& G = a.G;
   const int ti = tile.ti, tj = tile.tj, tk = tile.tk;
   const int tid = (int)threadIdx.y * TI + (int)threadIdx.x;
-  const int lane = tid % TIL, row = tid / TIL;
+  // (a strip's waves each hold one r column: uniform stencil rows per wave)
+  const int lane = TIL == TI ? tid % TIL : tid / TJ, row = TIL == TI ? tid / TIL : tid % TJ;
   // (a strip's ti is the count of whole r tiles: it starts where they end)
   const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
   const int ii = i0 + lane, jj = j0 + row;
