@@ -155,8 +155,40 @@ struct Combine {
   static constexpr bool boxed = !(MODE == M_PIRK_1A || MODE == M_PIRK_1B || MODE == M_PIRK3_2A || MODE == M_PIRK3_2B);
 };
 
-template <int MODE, int EF, int TJ>
-__device__ __forceinline__ void stage_body(const StageArgs& a, const Tile tile, const int sysidx) {
+// LDS of a 2nd-order tile of TIL x TJL points: 4 r-arrays and 2 theta-arrays
+// with their halos
+template <int TIL, int TJL>
+constexpr int lds2_doubles() {
+  return 4 * TJL * (TIL + 2 * HALO_R) + 2 * (TJL + 2) * TIL;
+}
+// the last r columns of a box as strips of SI2 x (TJ * TI / SI2) points
+// (one strip covers 128 theta rows at TJ = 8, 64 at 4), as the fd4 set does;
+// the RK stages only: on 2048 x 128 x 256 strips take the RK4 step from 7.84
+// to 7.62 ms but the PIRK2 step from 3.61 to 3.68 (its stage 2 alone 1.62 ->
+// 1.68 ms)
+constexpr int SI2 = 4;
+__host__ __device__ constexpr bool strips2(int mode) {
+  return mode == M_RK4_S1 || mode == M_RK4_S2 || mode == M_RK4_S3 || mode == M_RK4_S4 || mode == M_RK3_S1 ||
+         mode == M_RK3_S2 || mode == M_RK3_S3;
+}
+
+// LDS of a stage with strips: tiles and strips carve their arrays from one
+// buffer (with the tiles' own arrays beside a strip buffer, RK4 stage 4
+// takes 176 instead of 168 VGPRs and loses a wave per SIMD: 7.62 -> 7.90 ms
+// per RK4 step)
+template <int TJ>
+constexpr int lds2_of() {
+  return lds2_doubles<TI, TJ>() > lds2_doubles<SI2, TJ * TI / SI2>() ? lds2_doubles<TI, TJ>()
+                                                                      : lds2_doubles<SI2, TJ * TI / SI2>();
+}
+
+// TIL x TJL: the tile's points in r and theta (TI x TJ, or a strip's SI2 x
+// TJ*TI/SI2), on the same TI*TJ threads.  The stages with strips carve the
+// LDS arrays from `lds`; the others declare their own (which leaves out the
+// arrays a stage of phi and psi alone, or of Pi alone, does not use).
+template <int MODE, int EF, int TJ, int TIL = TI, int TJL = TJ>
+__device__ __forceinline__ void stage_body(const StageArgs& a, const Tile tile, const int sysidx, double* lds) {
+  static_assert(TIL * TJL == TI * TJ, "a tile has one point per thread");
   constexpr bool N14 = ModeTraits<MODE>::n14;
   constexpr bool N5 = ModeTraits<MODE>::n5;
   constexpr int CLO = Combine<MODE>::lo, CHI = Combine<MODE>::hi;
@@ -168,19 +200,38 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const Tile tile, 
   // against reading u_p and k1 at the point again after the RHS)
   constexpr bool KEEP = Combine<MODE>::keep;
 
-  constexpr int W = TI + 2 * HALO_R;
-  __shared__ double sG[TJ][W];    // alpha*Pi
-  __shared__ double sPR[TJ][W];   // psi_r
-  __shared__ double sFR[TJ][W];   // r-flux of rhs_pi
-  __shared__ double sPS[TJ][W];   // Pi*sqrtg
-  __shared__ double sF[TJ + 2][TI];   // alpha*Pi + beta*psi_r
-  __shared__ double sFT[TJ + 2][TI];  // theta-flux of rhs_pi
+  constexpr int W = TIL + 2 * HALO_R;
+  double(*sG)[W];    // alpha*Pi
+  double(*sPR)[W];   // psi_r
+  double(*sFR)[W];   // r-flux of rhs_pi
+  double(*sPS)[W];   // Pi*sqrtg
+  double(*sF)[TIL];  // alpha*Pi + beta*psi_r
+  double(*sFT)[TIL];  // theta-flux of rhs_pi
+  if constexpr (!strips2(MODE)) {
+    __shared__ double aG[TJL][W], aPR[TJL][W], aFR[TJL][W], aPS[TJL][W];
+    __shared__ double aF[TJL + 2][TIL], aFT[TJL + 2][TIL];
+    sG = aG;
+    sPR = aPR;
+    sFR = aFR;
+    sPS = aPS;
+    sF = aF;
+    sFT = aFT;
+  } else {
+    sG = reinterpret_cast<double(*)[W]>(lds);
+    sPR = reinterpret_cast<double(*)[W]>(lds + TJL * W);
+    sFR = reinterpret_cast<double(*)[W]>(lds + 2 * TJL * W);
+    sPS = reinterpret_cast<double(*)[W]>(lds + 3 * TJL * W);
+    sF = reinterpret_cast<double(*)[TIL]>(lds + 4 * TJL * W);
+    sFT = reinterpret_cast<double(*)[TIL]>(lds + 4 * TJL * W + (TJL + 2) * TIL);
+  }
 
   const Geom& G = a.G;
   const int ti = tile.ti, tj = tile.tj, tk = tile.tk;
-  const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
-  const int tid = row * TI + lane;
-  const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJ;
+  const int tid = (int)threadIdx.y * TI + (int)threadIdx.x;
+  // (a strip's waves each hold one r column: uniform stencil rows per wave)
+  const int lane = TIL == TI ? tid % TI : tid / TJL, row = TIL == TI ? tid / TI : tid % TJL;
+  // (a strip's ti is the count of whole r tiles: it starts where they end)
+  const int i0 = a.i_lo + ti * TI, j0 = a.j_lo + tj * TJL;
   const int ii = i0 + lane, jj = j0 + row;
   const int ni = G.ni, nj = G.nj, Nz = G.Nz, Ny = G.Ny, g = G.g, ld = G.ld;
   const uint32_t plane8 = (uint32_t)(G.plane * 8);
@@ -299,21 +350,24 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const Tile tile, 
       return asg * (grp * psr + gpp * psp);
   };
 
-  // halo task of this thread: 1 = theta row, 2 = r column, 0 = none
+  // halo task of this thread: 1 = theta row, 2 = r column (two each side;
+  // a strip, whose columns reach ni-1, the left two only), 0 = none
+  constexpr int RH = TIL == TI ? 4 : 2;
+  static_assert(2 * TIL + RH * TJL <= TI * TJ, "one halo task per thread at most");
   int hkind = 0, hi = 0, hj = 0, hrow = 0, hcol = 0;
-  if (tid < 2 * TI) {
+  if (tid < 2 * TIL) {
     hkind = 1;
-    hi = i0 + (tid % TI);
-    hj = tid < TI ? j0 - 1 : j0 + TJ;
-    hrow = tid < TI ? 0 : TJ + 1;
-    hcol = tid % TI;
-  } else if (tid < 2 * TI + 4 * TJ) {
-    const int u = tid - 2 * TI;
-    const int cc = u % 4;
+    hi = i0 + (tid % TIL);
+    hj = tid < TIL ? j0 - 1 : j0 + TJL;
+    hrow = tid < TIL ? 0 : TJL + 1;
+    hcol = tid % TIL;
+  } else if (tid < 2 * TIL + RH * TJL) {
+    const int u = tid - 2 * TIL;
+    const int cc = u % RH;
     hkind = 2;
-    hj = j0 + u / 4;
-    hrow = u / 4;
-    hi = cc == 0 ? i0 - 2 : cc == 1 ? i0 - 1 : cc == 2 ? i0 + TI : i0 + TI + 1;
+    hj = j0 + u / RH;
+    hrow = u / RH;
+    hi = cc == 0 ? i0 - 2 : cc == 1 ? i0 - 1 : cc == 2 ? i0 + TIL : i0 + TIL + 1;
     hcol = hi - i0 + HALO_R;
   }
   if (hi < 0 || hi >= ni || hj < 0 || hj >= nj) hkind = 0;
@@ -483,7 +537,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const Tile tile, 
     // D_theta on an LDS theta-array (finite_differences.f90:214-230); the
     // pole row reads plane k_target and the last row may reach below the
     // tile, both from global memory through `far`.
-    auto Dth = [&](const double(*A)[TI], auto far) {
+    auto Dth = [&](const double(*A)[TIL], auto far) {
       if constexpr (!BOXED) {
         if (jj == 0) return (-A[row + 2][lane] + far(2, kt)) * rt;
         if (jj == nj - 1) {
@@ -1196,7 +1250,18 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
 // against one 8-row workgroup on the 2048 x 128 x 256 grid).
 template <int MODE, int EF, int TJ>
 __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 3 : MODE == M_RK4_S3 ? 4 : 1)) void stage_kernel(const StageArgs a) {
-  stage_body<MODE, EF, TJ>(a, tile_dealt((int)blockIdx.x, a.nti, a.ntj, a.ntk, P2), (int)blockIdx.y);
+  // workgroups [0, nts*ntk): the strips of the last r columns (RK stages);
+  // then the tiles, dealt over the XCDs
+  const int b = (int)blockIdx.x, ns = a.nts * a.ntk;
+  if constexpr (strips2(MODE)) {
+    __shared__ double lds[lds2_of<TJ>()];
+    if (b < ns)
+      stage_body<MODE, EF, TJ, SI2, TJ * TI / SI2>(a, Tile{a.nti, b % a.nts, b / a.nts}, (int)blockIdx.y, lds);
+    else
+      stage_body<MODE, EF, TJ>(a, tile_dealt(b - ns, a.nti, a.ntj, a.ntk, P2), (int)blockIdx.y, lds);
+  } else {
+    stage_body<MODE, EF, TJ>(a, tile_dealt(b, a.nti, a.ntj, a.ntk, P2), (int)blockIdx.y, nullptr);
+  }
 }
 
 // Many grids in one launch: grid q owns the workgroups [tstart[q], tstart[q+1])
@@ -1226,7 +1291,12 @@ __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 
     const StageArgs& a = descs[q];
     const int local = b - tstart[q];
     const int nt = a.nti * a.ntj * a.ntk;
-    stage_body<MODE, EF, TJ>(a, tile_in_run(local % nt, a.nti, a.ntj, a.ntk, P2), local / nt);
+    if constexpr (strips2(MODE)) {
+      __shared__ double lds[lds2_doubles<TI, TJ>()];
+      stage_body<MODE, EF, TJ>(a, tile_in_run(local % nt, a.nti, a.ntj, a.ntk, P2), local / nt, lds);
+    } else {
+      stage_body<MODE, EF, TJ>(a, tile_in_run(local % nt, a.nti, a.ntj, a.ntk, P2), local / nt, nullptr);
+    }
   }
 }
 
@@ -1568,10 +1638,10 @@ hipError_t launch_fd4(const StageArgs& a0, hipStream_t s, int nsys) {
   StageArgs a = a0;
   a.tj = TJ4;
   finalize_tiles(a);
-  // the last r tile as strips when its columns end at ni-1 (no right halo)
-  // and fewer than SI4 of them are in the box
+  // the last r tile as strips when its columns reach ni-1 within SI4 points
+  // (the strip then needs no right halo)
   const int i_last = a.i_lo + (a.nti - 1) * TI;
-  if (a.nti > 1 && a.i_hi == a.G.ni - 1 && a.i_hi - i_last < SI4) {
+  if (a.nti > 1 && a.G.ni - i_last <= SI4) {
     a.nti -= 1;
     a.nts = (a.j_hi - a.j_lo + SJ4 - 1) / SJ4;
   }
@@ -1585,8 +1655,17 @@ hipError_t launch_fd4(const StageArgs& a0, hipStream_t s, int nsys) {
 }
 
 template <int MODE, int TJ>
-hipError_t launch_mode_tj(const StageArgs& a, hipStream_t s, int nsys) {
-  dim3 grid((unsigned)(a.nti * a.ntj * a.ntk), (unsigned)nsys, 1);
+hipError_t launch_mode_tj(const StageArgs& a0, hipStream_t s, int nsys) {
+  StageArgs a = a0;
+  // the last r tile as strips when its columns reach ni-1 within SI2 points
+  // (the strip then needs no right halo)
+  const int i_last = a.i_lo + (a.nti - 1) * TI;
+  if (strips2(MODE) && a.nti > 1 && a.G.ni - i_last <= SI2) {
+    constexpr int TJS = TJ * TI / SI2;
+    a.nti -= 1;
+    a.nts = (a.j_hi - a.j_lo + TJS - 1) / TJS;
+  }
+  dim3 grid((unsigned)((a.nti * a.ntj + a.nts) * a.ntk), (unsigned)nsys, 1);
   dim3 block(TI, TJ, 1);
   if (a.G.metric_ef)
     hipLaunchKernelGGL((stage_kernel<MODE, 1, TJ>), grid, block, 0, s, a);

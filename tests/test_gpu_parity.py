@@ -78,6 +78,41 @@ def test_steps_match_oracle(scheme, name, p):
     gpu.close()
 
 
+# boxes whose last r tile holds 1-4 columns (the stage kernels then run those
+# columns as strips of 4 r points by 64 or 128 theta rows): the boxed stages
+# cover Nx+1 columns, the PIRK stages of the full box Nx+3; two strips on the
+# 139-row box
+STRIP_CASES = [
+    ("r_strip_full2", dict(Nx=127, Ny=9, Nz=9, xmin=1.8, xmax=25.0, metric="Kerr_Schild", a=0.3, M_bh=1.0, m=0.2,
+                           bc="outflow")),
+    ("r_strip_box1_full3", dict(Nx=128, Ny=8, Nz=10, xmin=1.8, xmax=25.0, metric="Kerr_Schild", a=0.0, M_bh=1.0,
+                                m=0.1, bc="inflow")),
+    ("r_strip_box2_full4", dict(Nx=129, Ny=10, Nz=9, xmin=2.5, xmax=25.0, metric="Schw_EF", a=0.0, M_bh=1.0, m=0.2,
+                                bc="sommerfeld")),
+    ("r_strip_box3", dict(Nx=130, Ny=7, Nz=11, xmin=1.8, xmax=25.0, metric="Kerr_Schild", a=0.5, M_bh=1.0, m=0.0,
+                          bc="outflow")),
+    ("r_strips_tall", dict(Nx=64, Ny=140, Nz=7, xmin=1.8, xmax=12.0, metric="Kerr_Schild", a=0.5, M_bh=1.0, m=0.3,
+                           bc="sommerfeld")),
+]
+
+
+@pytest.mark.parametrize("tj", [4, 8])
+@pytest.mark.parametrize("scheme", ["pirk2", "pirk3", "rk3", "rk4"])
+@pytest.mark.parametrize("name,p", STRIP_CASES, ids=[c[0] for c in STRIP_CASES])
+def test_r_strips_match_oracle(tj, scheme, name, p):
+    print("strip case", name, scheme, tj, flush=True)
+    sim, gpu, _ = _pair(p, scheme, seed=2)
+    gpu.set_tuning(8, tj)
+    nsteps = 5
+    sim.step(scheme, nsteps)
+    gpu.step(nsteps)
+    got = gpu.u_soa()[0]
+    assert np.array_equal(got, sim.u), (name, scheme, tj, _rel_err(got, sim.u))
+    if scheme in ("pirk2", "pirk3"):
+        assert np.array_equal(gpu.u_stage_soa(0)[0], sim.u_stage)
+    gpu.close()
+
+
 @pytest.mark.parametrize("scheme", ["pirk2", "rk4"])
 def test_single_step_entry_points(scheme):
     """MoL%pirk2()/MoL%rk4() called one step at a time equal kg_step (graph replay)."""
