@@ -167,10 +167,10 @@ kg_status kg_batch_step(kg_handle* const* hs, int32_t n, int32_t nsteps);
  * device array of the library itself, 4 MiB per GPU (held until the process
  * ends; its address is resolved at kg_create), instead of a buffer the lead
  * handle grows; the call makes no hipMalloc/hipFree and creates no event.
- * The descriptors take 2 x (launches per step) x n x 496 bytes plus small
- * prefix arrays (PIRK2: 4 launches per step, about 1040 members at most;
- * RK4: 5, about 830; RK3 and PIRK3: 6, about 690; with KG_BC_SOMMERFELD a
- * stage adds its outer-column launch: RK4 about 460): a call that needs more
+ * The descriptors take 2 x (launches per step) x n x 504 bytes plus small
+ * prefix arrays (PIRK2: 4 launches per step, about 1020 members at most;
+ * RK4, RK3 and PIRK3: 6, about 680; with KG_BC_SOMMERFELD a stage adds its
+ * outer-column launch: RK4 about 410): a call that needs more
  * returns KG_ERR_INVALID.
  * Every member's stage buffers must exist (kg_create makes those of its
  * desc.scheme).  Kernel timing (kg_profile_enable) is not recorded.  All
