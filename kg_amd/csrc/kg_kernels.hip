@@ -1241,11 +1241,12 @@ __global__ __launch_bounds__(TI* TJ4) void stage4_kernel_batch(const StageArgs* 
   }
 }
 
-// One grid: blockIdx.x = tile (XCD-remapped), blockIdx.y = system.  The
-// second launch bound is the minimum number of waves per SIMD.  RK4 stage 3
-// asks for 4 (128 VGPRs), stage 2 for 3 (~164 VGPRs with u_p and k1 kept for
-// its stores) and the final stage for 2 (its natural ~184 VGPRs; 3 or 4 waves
-// spill); stages 2 and 4 always run 4 theta rows per workgroup, so several
+// One grid: blockIdx.x = strip or tile, blockIdx.y = system.  The second
+// launch bound is the minimum number of waves per SIMD.  RK4 stage 3 asks for
+// 4 (128 VGPRs), stage 2 for 3 (~157-164 VGPRs with u_p and k1 kept for its
+// stores) and the final stage for 2 (its natural count: 168 VGPRs now, which
+// fits 3; at ~184 a bound of 3 or 4 spilled); stages 2 and 4 always run 4
+// theta rows per workgroup, so several
 // workgroups with their own barriers share a CU (stage 4: 3.25 -> 3.14 ms
 // against one 8-row workgroup on the 2048 x 128 x 256 grid).
 template <int MODE, int EF, int TJ>
