@@ -34,6 +34,8 @@ namespace {
 // (measured 7-15 % on a batched stage launch).
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) const double gcdouble;
+typedef double double2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) double2v gdouble2;
 typedef __attribute__((address_space(1))) char gchar;
 typedef __attribute__((address_space(1))) const char gcchar;
 
@@ -1308,8 +1310,8 @@ __global__ __launch_bounds__(TI* TJ, (MODE == M_RK4_S4 ? 2 : MODE == M_RK4_S2 ? 
 // Work items of each fill pass of one grid (all fields and systems).
 __host__ __device__ inline int64_t fill_items(const Geom& G, int nsys, int phase) {
   if (phase == 0) return (int64_t)G.nj * G.nk * 5 * nsys;
-  if (phase == 1) return (int64_t)G.ni * 2 * (G.g + 1) * G.nk * 5 * nsys;
-  return G.plane * 2 * 5 * nsys;
+  if (phase == 1) return (int64_t)((G.ni + 1) / 2) * 2 * (G.g + 1) * G.nk * 5 * nsys;
+  return G.plane / 2 * 2 * 5 * nsys;
 }
 
 // A work item's index is split with the index type I: uint32_t whenever the
@@ -1339,10 +1341,15 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, I e) {
     }
   } else if constexpr (PHASE == 1) {
     // boundary_yL (:542-555) and boundary_yR (:565-578): whole r-lines,
-    // rows -gg <- gg+1 and Ny+gg <- Ny-gg-1, taken at k_target = mod(k+Nz/2, Nz-1)
+    // rows -gg <- gg+1 and Ny+gg <- Ny-gg-1, taken at k_target = mod(k+Nz/2, Nz-1);
+    // an item copies two neighbouring points of a line (rows start 128-byte
+    // aligned: ld, the plane and the field stride are multiples of 16
+    // doubles), the last odd point alone (the pass 69 -> 48 us at 2048 x 128
+    // x 256)
     const int nrow = G.g + 1;
-    const int ii = (int)(e % (I)G.ni);
-    I r = e / (I)G.ni;
+    const int hp = (G.ni + 1) / 2;
+    const int ii = 2 * (int)(e % (I)hp);
+    I r = e / (I)hp;
     const int row = (int)(r % (I)(2 * nrow));
     r /= (I)(2 * nrow);
     const int kk = (int)(r % (I)G.nk);
@@ -1360,20 +1367,27 @@ __device__ __forceinline__ void fill_item(const FillArgs& a, I e) {
       js = G.g + G.Ny - gg - 1;   // j = Ny-gg-1
     }
     gdouble* u = gptr(a.u[c]) + sys;
-    u[(int64_t)kk * G.plane + (int64_t)jd * G.ld + ii] = u[(int64_t)kt * G.plane + (int64_t)js * G.ld + ii];
+    const int64_t dst = (int64_t)kk * G.plane + (int64_t)jd * G.ld + ii;
+    const int64_t src = (int64_t)kt * G.plane + (int64_t)js * G.ld + ii;
+    if (ii + 1 < G.ni)
+      *(gdouble2*)(u + dst) = *(gdouble2*)(u + src);
+    else
+      u[dst] = u[src];
   } else {
-    // boundary_zL (:587) and boundary_zR (:596)
-    const int64_t t = (int64_t)(e % (I)G.plane);
-    I r = e / (I)G.plane;
+    // boundary_zL (:587) and boundary_zR (:596); an item copies two
+    // neighbouring points (the plane is a multiple of 16 doubles)
+    const int64_t hp = G.plane / 2;
+    const int64_t t = 2 * (int64_t)(e % (I)hp);
+    I r = e / (I)hp;
     const int side = (int)(r % 2);
     r /= 2;
     const int c = (int)(r % 5);
     const int64_t sys = (int64_t)(r / 5) * G.sys_stride;
     gdouble* u = gptr(a.u[c]) + sys;
     if (side == 0)
-      u[t] = u[(int64_t)(G.Nz - 1) * G.plane + t];
+      *(gdouble2*)(u + t) = *(gdouble2*)(u + (int64_t)(G.Nz - 1) * G.plane + t);
     else
-      u[(int64_t)G.Nz * G.plane + t] = u[G.plane + t];
+      *(gdouble2*)(u + (int64_t)G.Nz * G.plane + t) = *(gdouble2*)(u + G.plane + t);
   }
 }
 
