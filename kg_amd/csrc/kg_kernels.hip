@@ -1726,8 +1726,8 @@ void finalize_tiles(StageArgs& a) {
   a.ntj = (a.j_hi - a.j_lo + a.tj - 1) / a.tj;
   a.nts = 0;
   a.ntk = (a.k_hi - a.k_lo + a.kc - 1) / a.kc;
-  // the same number of chunks, balanced: 257 planes at kc = 32 march as 9
-  // chunks of 29 instead of 8 of 32 and one of a single plane
+  // the same number of chunks, balanced: a full box of 257 planes at kc = 32
+  // marches as 9 chunks of 29 instead of 8 of 32 and one of a single plane
   a.kc = (a.k_hi - a.k_lo + a.ntk - 1) / a.ntk;
   a.ntk = (a.k_hi - a.k_lo + a.kc - 1) / a.kc;
 }
