@@ -709,11 +709,11 @@ constexpr int TJ4 = 8;
 // SJ4 points instead of 64 x 8 tiles with a few active lanes: one strip
 // covers 128 theta rows, where the 64-wide tiles of those columns took 16
 // workgroups.  On 2048 x 128 x 256 those tiles cost 5.8 % of an RK4 step;
-// the strips take 8.59 -> 8.47 ms per step back (a strip's waves span 16
-// theta rows, each lane with its own fixed r row, so its march is several
-// times slower than a tile's).  A strip needs no right r halo (its columns
-// reach ni-1), so it is used when its r range holds at most SI4 - 1
-// columns of the box.
+// the strips take 8.59 -> 8.41 ms per step back (a strip's march is several
+// times slower than a tile's, and the 4096 tiles being 16 rounds of 256,
+// its work puts one tile-march more on the launch's last round).  A strip
+// needs no right r halo (its columns reach ni-1), so it is used when its r
+// range holds at most SI4 - 1 columns of the box.
 constexpr int SI4 = 4, SJ4 = 128;
 
 // EDGE: the tile's r range reaches a fixed row of x_4 (1, N-1) or of
