@@ -493,6 +493,8 @@ def main():
                     help="timed steps of the other operator set (0: skip)")
     ap.add_argument("--pirk2-steps", type=int, default=50,
                     help="timed steps of the reference's PIRK2 with its 2nd-order stencils on the full grid (0: skip)")
+    ap.add_argument("--pirk2-kc", type=int, default=0,
+                    help="fix the PIRK2 leg's phi planes per workgroup (8 theta rows; skips its autotune)")
     ap.add_argument("--kc", type=int, default=0, help="fix the stage kernel's phi planes per workgroup (skips autotune)")
     ap.add_argument("--tj", type=int, default=8, help="theta rows per workgroup when --kc is given")
     ap.add_argument("--batched", type=int, default=256,
@@ -559,7 +561,7 @@ def main():
         out["plain_" + other] = plain(other, args.other_steps, 5)
     if args.pirk2_steps > 0:
         print("bench: the reference's PIRK2 + 2nd-order step", file=sys.stderr, flush=True)
-        out["plain_ref2_pirk2"] = plain("ref2", args.pirk2_steps, 5, scheme="pirk2")
+        out["plain_ref2_pirk2"] = plain("ref2", args.pirk2_steps, 5, kc=args.pirk2_kc, tj=8, scheme="pirk2")
     if args.refined_steps > 0:
         out["refined_pirk"] = [refined_workload(sch, args.refined_steps) for sch in ("pirk2", "pirk3", "rk3")]
     if args.ic > 0:
